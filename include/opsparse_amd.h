@@ -1,0 +1,217 @@
+/*
+ * opsparse_amd: C ABI of the MI355X (gfx950) SpGEMM library, C = A * B in CSR.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never
+ * throws, and returns a status code.  Device arrays are HIP device pointers;
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *
+ * The reference (hongyx11/opsparse) exposes C++ entry points only; each
+ * function below names the reference interface it replaces.  The C++
+ * drop-in (namespace opsparse, include/opsparse/...) is built on top of it.
+ */
+#ifndef OPSPARSE_AMD_H_
+#define OPSPARSE_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  0-5 keep the values of the reference's SpgemmStatus
+ * (include/opsparse/profile/perf.hpp:16-23); 6 is new (bad arguments). */
+enum {
+    OPS_OK = 0,
+    OPS_NONSQUARE = 1,      /* A.N != B.M */
+    OPS_INT32_OVERFLOW = 2, /* total products or nnz(C) above INT32_MAX on an int32 output */
+    OPS_OOM = 3,            /* hipMalloc failed */
+    OPS_HIP_ERROR = 4,      /* any other HIP runtime error */
+    OPS_EXCEPTION = 5,      /* internal exception caught at the boundary */
+    OPS_INVALID = 6         /* invalid argument (null pointer, negative size, bad dtype) */
+};
+
+typedef enum { OPS_F64 = 0, OPS_F32 = 1 } ops_dtype;
+
+/* Device CSR, int32 offsets: the device mirror of opsparse::CSR
+ * (include/opsparse/core_ds/csr.hpp:12-21: M, N, nnz, d_rpt, d_col, d_val).
+ * rpt[r]..rpt[r+1] index col/val directly, so rpt need not start at 0
+ * (a row slice of a larger CSR is a valid descriptor). */
+typedef struct ops_csr {
+    int32_t M;
+    int32_t N;
+    int32_t nnz;
+    int32_t *rpt; /* M+1 */
+    int32_t *col; /* nnz, strictly ascending per row */
+    void *val;    /* nnz, double (OPS_F64) or float (OPS_F32) */
+} ops_csr;
+
+/* Output CSR with 64-bit row offsets, for products whose nnz(C) exceeds
+ * INT32_MAX (the reference refuses these: spgemm.cu:173-177, 210-213). */
+typedef struct ops_csr64 {
+    int64_t M;
+    int64_t N;
+    int64_t nnz;
+    int64_t *rpt; /* M+1 */
+    int32_t *col;
+    void *val;
+} ops_csr64;
+
+#define OPS_NUM_BINS 8
+
+/* Per-phase timings in seconds (the fields of opsparse::SpgemmTimings,
+ * include/opsparse/profile/spgemm_timings.hpp:8-18), plus per-bin kernel
+ * statistics used for the roofline report. */
+typedef struct ops_timings {
+    int measure_separate; /* in: 1 = synchronise after every phase and time it on the host clock */
+    int kernel_events;    /* in: 1 = time every per-bin kernel with HIP events on its own stream */
+    double setup;
+    double symbolic_binning;
+    double symbolic;
+    double reduce;
+    double numeric_binning;
+    double prefix;
+    double allocate;
+    double numeric;
+    double cleanup;
+    double total;
+    /* out: problem statistics */
+    int64_t total_flop; /* intermediate products P */
+    int64_t nnz_c;      /* nnz(C) */
+    int32_t max_row_flop;
+    int32_t max_row_nnz;
+    /* out: per-bin rows, products and event-timed kernel milliseconds
+     * (ms are 0 unless kernel_events was set) */
+    int64_t sym_rows[OPS_NUM_BINS];
+    int64_t sym_flop[OPS_NUM_BINS];
+    double sym_ms[OPS_NUM_BINS];
+    int64_t num_rows[OPS_NUM_BINS];
+    int64_t num_flop[OPS_NUM_BINS];
+    int64_t num_nnz[OPS_NUM_BINS];
+    int64_t num_arows_nnz[OPS_NUM_BINS]; /* nnz of A over the rows of the bin */
+    double num_ms[OPS_NUM_BINS];
+} ops_timings;
+
+/* Persistent per-device workspace: stream pool, events, growable device
+ * buffers.  Replaces opsparse::SpgemmMeta (include/opsparse/core_algo/spgemm/meta.hpp:15-60),
+ * which the reference allocates and frees inside every call (meta.cu:16-65). */
+typedef struct ops_workspace ops_workspace;
+
+int ops_workspace_create(ops_workspace **ws);
+int ops_workspace_destroy(ops_workspace *ws);
+/* Last error message recorded on this workspace (never NULL). */
+const char *ops_workspace_last_error(const ops_workspace *ws);
+
+/*
+ * C = A * B.  Replaces opsparse::opsparse_spgemm / opsparse_spgemm_reg
+ * (include/opsparse/core_algo/spgemm/spgemm.hpp:12-17; src/opsparse/core_algo/spgemm/spgemm.cu:16-126).
+ * The library allocates C->rpt, C->col and C->val with hipMalloc (free them
+ * with ops_csr_free, the counterpart of CSR::release, csr.cu:40-56) and sets
+ * C->M, C->N, C->nnz.  Returns when C is complete on `stream`.  A and B must
+ * have sorted, duplicate-free rows; A->N must equal B->M.
+ */
+int ops_spgemm_f64(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *C, void *stream, ops_timings *t);
+int ops_spgemm_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *C, void *stream, ops_timings *t);
+
+/* C = A * B with 64-bit row offsets for C (no counterpart in the reference,
+ * which returns int32_overflow instead: spgemm.cu:163-177). */
+int ops_spgemm64_f64(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr64 *C, void *stream, ops_timings *t);
+int ops_spgemm64_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr64 *C, void *stream, ops_timings *t);
+
+/* Frees library-allocated C arrays and zeroes the descriptor (CSR::drelease, csr.cu:40-48). */
+int ops_csr_free(ops_csr *C);
+int ops_csr64_free(ops_csr64 *C);
+
+/*
+ * Safe variant: never throws, classifies failures.  Replaces
+ * opsparse::opsparse_spgemm_safe (spgemm.cu:148-219).  Reads the HOST arrays
+ * a_rpt_host, a_col_host and b_rpt_host for the int32 pre-check exactly as
+ * the reference does (compute_flop.cu:59-65); returns OPS_NONSQUARE or
+ * OPS_INT32_OVERFLOW before any GPU work.  total_flop_out / nnz_c_out may be
+ * NULL; free_before / free_after receive hipMemGetInfo snapshots.
+ */
+int ops_spgemm_safe_f64(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const int32_t *a_rpt_host,
+                        const int32_t *a_col_host, const int32_t *b_rpt_host, ops_csr *C, void *stream,
+                        ops_timings *t, int64_t *total_flop_out, int64_t *nnz_c_out, size_t *free_before,
+                        size_t *free_after, size_t *total_bytes);
+
+/* Host product count with OpenMP.  Replaces opsparse::compute_flop
+ * (include/opsparse/numeric/compute_flop.hpp:11-13; compute_flop.cu:33-65);
+ * row_flop_out (M entries) may be NULL.  Returns the 64-bit total. */
+int64_t ops_compute_flop_host(int32_t M, const int32_t *a_rpt, const int32_t *a_col, const int32_t *b_rpt,
+                              int64_t *row_flop_out);
+
+/* Device product count: row_flop[M] (int32, saturated) and the 64-bit total.
+ * Replaces k_compute_flop (compute_flop.cu:9-31). */
+int ops_compute_flop_device(ops_workspace *ws, const ops_csr *A, const ops_csr *B, int32_t *row_flop,
+                            int64_t *total_out, void *stream);
+
+/* ---- Structure reuse: plan once, run the numeric phase many times ------- */
+
+/* A plan holds C's row offsets and the numeric row bins for fixed patterns
+ * of A and B (no counterpart in the reference).  ops_plan_create runs the
+ * setup, symbolic and scan phases and reports nnz(C). */
+typedef struct ops_plan ops_plan;
+
+int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_plan **plan, int64_t *nnz_c,
+                    void *stream);
+int ops_plan_destroy(ops_plan *plan);
+/* Copies C's row offsets (M+1) into caller device memory, int32 or int64. */
+int ops_plan_rowptr32(const ops_plan *plan, int32_t *rpt, void *stream);
+int ops_plan_rowptr64(const ops_plan *plan, int64_t *rpt, void *stream);
+/* Numeric phase into caller-owned device buffers ccol[nnz_c], cval[nnz_c]:
+ * no allocation and no host synchronisation, so it can be captured in a
+ * hipGraph.  A and B must have the patterns the plan was made with. */
+int ops_plan_numeric(ops_plan *plan, ops_dtype dt, const ops_csr *A, const ops_csr *B, int32_t *ccol, void *cval,
+                     void *stream);
+/* Captures ops_plan_numeric into a hipGraph on first use and replays it. */
+int ops_plan_numeric_graph(ops_plan *plan, ops_dtype dt, const ops_csr *A, const ops_csr *B, int32_t *ccol,
+                           void *cval, void *stream);
+
+/* ---- Masked backward at fixed sparsity (no counterpart in the reference) -- */
+
+/* Given dC on pattern(C) (ccol sorted per row, row offsets crpt64),
+ * dA = (dC * B^T) sampled on pattern(A), dB = (A^T * dC) sampled on
+ * pattern(B).  da/db are overwritten (nnz(A) / nnz(B) values). */
+int ops_spgemm_backward(ops_workspace *ws, ops_dtype dt, const ops_csr *A, const ops_csr *B, const int64_t *crpt64,
+                        const int32_t *ccol, const void *dcval, void *da, void *db, void *stream);
+
+/* ---- Synthetic inputs (host, deterministic across machines) ------------- */
+
+/* Host CSR produced by the generators; free with ops_host_csr_free. */
+typedef struct ops_host_csr {
+    int32_t M, N;
+    int64_t nnz;
+    int32_t *rpt;
+    int32_t *col;
+    double *val;
+} ops_host_csr;
+
+/* M x N, exactly d distinct uniform columns per row, values U[-1,1]. */
+int ops_gen_uniform_rows(int32_t M, int32_t N, int32_t d, uint64_t seed, ops_host_csr *out);
+/* 27-point stencil on nx*ny*nz (x fastest), diagonal `diag`, off-diagonal
+ * `off` + U[-noise, noise]. */
+int ops_gen_stencil27(int32_t nx, int32_t ny, int32_t nz, double diag, double off, double noise, uint64_t seed,
+                      ops_host_csr *out);
+/* R-MAT scale s (2^s x 2^s), edge_factor * 2^s edges drawn with (a,b,c,d),
+ * duplicates merged, no vertex permutation; values U[-1,1] (normal == 0) or
+ * N(0,1) (normal == 1) drawn after merging. */
+int ops_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed, int normal,
+                 ops_host_csr *out);
+/* Batch of `count` instances packed block-diagonally: instance i is n_i x n_i
+ * with n_i = floor(exp(U[ln n_lo, ln n_hi])), d_i uniform in [d_lo, d_hi]
+ * distinct uniform columns per row, values U[-1,1]; A and B are drawn
+ * independently.  row_base (count+1, host memory supplied by the caller)
+ * receives the instance offsets; d_out (count) the per-instance d. */
+int ops_gen_batch(int32_t count, int32_t n_lo, int32_t n_hi, int32_t d_lo, int32_t d_hi, uint64_t seed,
+                  ops_host_csr *A, ops_host_csr *B, int64_t *row_base, int32_t *d_out);
+int ops_host_csr_free(ops_host_csr *m);
+
+/* Library version string. */
+const char *ops_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPSPARSE_AMD_H_ */

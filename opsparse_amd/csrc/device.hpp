@@ -1,0 +1,317 @@
+// Device-side building blocks for the gfx950 SpGEMM kernels.
+//
+// Design notes (CDNA4, wave64):
+//  * A "unit" is an aligned group of U lanes (U = 8..64, power of two) that
+//    cooperates on one A-row chunk.  All cross-lane traffic inside a unit goes
+//    through ds_bpermute with explicit source lanes, so units of one wave
+//    never see each other and no LDS memory or barrier is needed for it.
+//  * Products of an A row are flattened: the unit loads U A-nonzeros, scans
+//    their B-row lengths, and lane l then takes products l, l+U, ... of the
+//    chunk, finding its A-nonzero by a binary search over the scanned
+//    offsets.  Every lane is busy whatever the B-row lengths are, which is
+//    what a 64-wide wavefront needs (the reference gives each A-nonzero a
+//    32-lane warp and strides over its B row, symbolic.cuh:94-96, which
+//    idles most lanes on short B rows).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+namespace opsamd {
+
+constexpr int kWave = 64;
+
+// Multiplicative hash constant of the reference (utils/config.hpp:32, kHashScale).
+constexpr unsigned kHashMul = 107u;
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(__lane_id()); }
+
+// Value of `v` held by absolute lane `src` of this wave.
+__device__ __forceinline__ int bperm(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+__device__ __forceinline__ unsigned bperm(unsigned v, int src)
+{
+    return static_cast<unsigned>(__builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(v)));
+}
+__device__ __forceinline__ float bperm(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ unsigned long long bperm(unsigned long long v, int src)
+{
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(v >> 32));
+    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
+}
+__device__ __forceinline__ long long bperm(long long v, int src)
+{
+    return static_cast<long long>(bperm(static_cast<unsigned long long>(v), src));
+}
+__device__ __forceinline__ double bperm(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(b));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// Inclusive prefix sum over aligned groups of U lanes.
+template <int U>
+__device__ __forceinline__ int unit_incl_scan(int v)
+{
+    const int lane = lane_id();
+    const int gl = lane & (U - 1);
+#pragma unroll
+    for (int d = 1; d < U; d <<= 1) {
+        const int t = bperm(v, (lane - d) & (kWave - 1));
+        if (gl >= d) v += t;
+    }
+    return v;
+}
+
+// Sum over aligned groups of U lanes (every lane gets the total).
+template <int U, typename T>
+__device__ __forceinline__ T unit_sum(T v)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = U >> 1; d >= 1; d >>= 1) v += bperm(v, lane ^ d);
+    return v;
+}
+
+template <int U>
+__device__ __forceinline__ int unit_min(int v)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = U >> 1; d >= 1; d >>= 1) v = min(v, bperm(v, lane ^ d));
+    return v;
+}
+
+// Ballot restricted to this lane's unit, shifted down to bit 0.
+template <int U>
+__device__ __forceinline__ unsigned long long unit_ballot(bool p)
+{
+    const unsigned long long m = __ballot(p);
+    if constexpr (U == 64) {
+        return m;
+    } else {
+        const int base = lane_id() & ~(U - 1);
+        return (m >> base) & ((1ull << U) - 1ull);
+    }
+}
+
+// Wavefront-scope ordering point for LDS traffic between lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Default B-row range of an A-nonzero: the whole B row.
+struct FullRow {
+    const int *__restrict__ brpt;
+    __device__ __forceinline__ void operator()(int /*e*/, int k, int &bs, int &len) const
+    {
+        bs = brpt[k];
+        len = brpt[k + 1] - bs;
+    }
+};
+
+// Calls f(b_index, a_value) for every product of A row segment [a_beg, a_end).
+// Unit `unit` of `nunits` units takes A chunks unit, unit+nunits, ...  All
+// lanes of a unit must call this together with the same arguments.
+// `range(e, k, bs, len)` gives the B entries [bs, bs+len) an A-nonzero e
+// (column k) contributes.
+template <int U, bool WITH_A, typename V, typename Range, typename F>
+__device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit, int nunits,
+                                                 const int *__restrict__ acol, const V *__restrict__ aval,
+                                                 const Range &range, F &&f)
+{
+    const int lane = lane_id();
+    const int gl = lane & (U - 1);
+    const int gbase = lane & ~(U - 1);
+    for (int c = a_beg + unit * U; c < a_end; c += nunits * U) {
+        const int e = c + gl;
+        int bs = 0, len = 0;
+        V av = V(0);
+        if (e < a_end) {
+            const int k = acol[e];
+            range(e, k, bs, len);
+            if constexpr (WITH_A) av = aval[e];
+        }
+        const int incl = unit_incl_scan<U>(len);
+        const int excl = incl - len;
+        const int total = bperm(incl, gbase + U - 1);
+        for (int base = 0; base < total; base += U) {
+            const int p = base + gl;
+            int q = 0, qv = 0;
+#pragma unroll
+            for (int s = U >> 1; s >= 1; s >>= 1) {
+                const int cand = q + s;
+                const int v = bperm(excl, gbase + cand);
+                if (v <= p) {
+                    q = cand;
+                    qv = v;
+                }
+            }
+            const int qb = bperm(bs, gbase + q);
+            V qa = V(0);
+            if constexpr (WITH_A) qa = bperm(av, gbase + q);
+            if (p < total) f(qb + (p - qv), qa);
+        }
+    }
+}
+
+// Linear-probing insert of `key` (>= 0) into an int table of T (power of 2)
+// slots initialised to -1.  Returns the slot; `fresh` tells whether this
+// call created the key.
+template <int T>
+__device__ __forceinline__ int hash_insert(int *tab, int key, bool &fresh)
+{
+    unsigned h = (static_cast<unsigned>(key) * kHashMul) & (T - 1);
+    while (true) {
+        const int old = atomicCAS(tab + h, -1, key);
+        if (old == -1) {
+            fresh = true;
+            return static_cast<int>(h);
+        }
+        if (old == key) {
+            fresh = false;
+            return static_cast<int>(h);
+        }
+        h = (h + 1) & (T - 1);
+    }
+}
+
+// Same with a runtime power-of-two mask.
+__device__ __forceinline__ int hash_insert_mask(int *tab, int key, unsigned mask, bool &fresh)
+{
+    unsigned h = (static_cast<unsigned>(key) * kHashMul) & mask;
+    while (true) {
+        const int old = atomicCAS(tab + h, -1, key);
+        if (old == -1) {
+            fresh = true;
+            return static_cast<int>(h);
+        }
+        if (old == key) {
+            fresh = false;
+            return static_cast<int>(h);
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// Lookup; returns -1 when the key is absent (the probe stops at an empty
+// slot or after T probes, so a mismatched pattern can never spin).
+template <int T>
+__device__ __forceinline__ int hash_find(const int *tab, int key)
+{
+    unsigned h = (static_cast<unsigned>(key) * kHashMul) & (T - 1);
+    for (int i = 0; i < T; i++) {
+        const int v = tab[h];
+        if (v == key) return static_cast<int>(h);
+        if (v == -1) return -1;
+        h = (h + 1) & (T - 1);
+    }
+    return -1;
+}
+
+// Bitonic sort of K*U (key, value) pairs held K per lane across a unit of U
+// lanes; element j = k*U + (lane % U) ends up holding the j-th smallest key.
+// Keys must be distinct except for INT_MAX padding.
+template <int U, int K, typename V>
+__device__ __forceinline__ void unit_bitonic_sort(int (&key)[K], V (&val)[K])
+{
+    const int lane = lane_id();
+    const int gl = lane & (U - 1);
+    constexpr int NEL = K * U;
+#pragma unroll
+    for (int size = 2; size <= NEL; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= U) {
+                const int ks = stride / U;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int pk = k ^ ks;
+                    if (pk > k) {
+                        const int j = k * U + gl;
+                        const bool asc = (j & size) == 0;
+                        if ((key[k] > key[pk]) == asc) {
+                            const int tk = key[k];
+                            key[k] = key[pk];
+                            key[pk] = tk;
+                            const V tv = val[k];
+                            val[k] = val[pk];
+                            val[pk] = tv;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int j = k * U + gl;
+                    const bool asc = (j & size) == 0;
+                    const bool lower = (j & stride) == 0;
+                    const int ok = bperm(key[k], lane ^ stride);
+                    const V ov = bperm(val[k], lane ^ stride);
+                    const bool take_min = (asc == lower);
+                    if (take_min ? (ok < key[k]) : (ok > key[k])) {
+                        key[k] = ok;
+                        val[k] = ov;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Block-wide exclusive scan of one int per thread; returns the exclusive
+// prefix and writes the block total.  `scratch` needs BS/64 + 1 ints of LDS.
+template <int BS>
+__device__ __forceinline__ int block_excl_scan(int v, int *scratch, int &total)
+{
+    const int lane = lane_id();
+    const int wid = threadIdx.x / kWave;
+    const int incl = unit_incl_scan<kWave>(v);
+    if (lane == kWave - 1) scratch[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int w = 0; w < BS / kWave; w++) {
+            const int t = scratch[w];
+            scratch[w] = run;
+            run += t;
+        }
+        scratch[BS / kWave] = run;
+    }
+    __syncthreads();
+    total = scratch[BS / kWave];
+    const int r = scratch[wid] + incl - v;
+    __syncthreads();
+    return r;
+}
+
+template <int BS, typename T>
+__device__ __forceinline__ T block_sum(T v, T *scratch)
+{
+    const int lane = lane_id();
+    const int wid = threadIdx.x / kWave;
+    v = unit_sum<kWave>(v);
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    T s = T(0);
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < BS / kWave; w++) s += scratch[w];
+        scratch[0] = s;
+    }
+    __syncthreads();
+    s = scratch[0];
+    __syncthreads();
+    return s;
+}
+
+}  // namespace opsamd

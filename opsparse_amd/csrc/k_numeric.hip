@@ -1,0 +1,411 @@
+// Numeric phase: values and sorted columns of every row of C.
+//
+// Reference counterparts: k_numeric_shared_hash_pwarp,
+// k_numeric_shared_hash_tb_full_occu<SH_ROW,BS>,
+// k_numeric_max_shared_hash_tb_half_occu, k_numeric_global_hash_tb_full_occu
+// (include/opsparse/core_algo/spgemm/kernels/numeric.cuh:9-393) and their
+// dispatch h_numeric_full_occu (src/opsparse/core_algo/spgemm/spgemm_phases.cu:148-206).
+//
+// The accumulation is the same open-addressing hash (CAS on the key, atomic
+// add of a*b on the value), with tables of >= 2n slots for a row of n
+// entries.  The ordering step is different: the reference ranks every entry
+// against every other entry, O(n^2) (numeric.cuh:97-106, 194-204, 382-392);
+// here rows of up to 256 entries are compacted and bitonic-sorted in
+// registers across their unit (cross-lane ds_bpermute), rows up to 4096 are
+// bitonic-sorted in LDS, and rows above 4096 entries never need a sort: they
+// accumulate into dense LDS column windows whose occupancy bitmap is read
+// out in column order.
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace opsamd {
+
+namespace {
+
+template <typename V>
+struct Prod {
+    const int *__restrict__ bcol;
+    const V *__restrict__ bval;
+};
+
+// Bins 0-4: G <= 64 lanes per row, BS/G rows per block, register sort.
+template <int G, int T, int BS, typename V>
+__global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                  const V *__restrict__ aval, const int *__restrict__ brpt,
+                                                  const int *__restrict__ bcol, const V *__restrict__ bval,
+                                                  const int *__restrict__ rows, int nrows,
+                                                  const long long *__restrict__ crpt, int *__restrict__ ccol,
+                                                  V *__restrict__ cval)
+{
+    constexpr int U = G;
+    constexpr int RPB = BS / G;
+    constexpr int S = T / U;        // table slots per lane
+    constexpr int K = T / (2 * U);  // sorted entries per lane (n <= T/2)
+    static_assert(G <= kWave && BS % G == 0 && K >= 1, "bad config");
+    __shared__ int keys[RPB * T];
+    __shared__ V vals[RPB * T];
+
+    const int tid = threadIdx.x;
+    const int slot = tid / G;
+    const int gl = tid % G;
+    for (int i = tid; i < RPB * T; i += BS) {
+        keys[i] = -1;
+        vals[i] = V(0);
+    }
+    __syncthreads();
+
+    const int ridx = blockIdx.x * RPB + slot;
+    int r = 0;
+    int *kt = keys + slot * T;
+    V *vt = vals + slot * T;
+    if (ridx < nrows) {
+        r = rows ? rows[ridx] : ridx;
+        const FullRow range{brpt};
+        for_each_product<U, true, V>(arpt[r], arpt[r + 1], 0, 1, acol, aval, range, [&](int k, V a) {
+            bool fresh;
+            const int h = hash_insert<T>(kt, bcol[k], fresh);
+            atomicAdd(vt + h, a * bval[k]);
+        });
+    }
+    __syncthreads();
+    if (ridx >= nrows) return;
+
+    // Compact the row's slots in place, in slot order.
+    int kk[S];
+    V vv[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        kk[s] = kt[s * U + gl];
+        vv[s] = vt[s * U + gl];
+    }
+    wave_lds_sync();
+    int n = 0;
+    const unsigned long long below = gl ? ((1ull << gl) - 1ull) : 0ull;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const bool valid = kk[s] != -1;
+        const unsigned long long m = unit_ballot<U>(valid);
+        if (valid) {
+            const int pos = n + __popcll(m & below);
+            kt[pos] = kk[s];
+            vt[pos] = vv[s];
+        }
+        n += __popcll(m);
+    }
+    wave_lds_sync();
+
+    int sk[K];
+    V sv[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = k * U + gl;
+        if (j < n) {
+            sk[k] = kt[j];
+            sv[k] = vt[j];
+        } else {
+            sk[k] = INT_MAX;
+            sv[k] = V(0);
+        }
+    }
+    unit_bitonic_sort<U, K, V>(sk, sv);
+    const long long off = crpt[r];
+    const int n_out = min(n, static_cast<int>(crpt[r + 1] - off));  // never write past the row
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = k * U + gl;
+        if (j < n_out) {
+            ccol[off + j] = sk[k];
+            cval[off + j] = sv[k];
+        }
+    }
+}
+
+// Bins 5-6: one row per block of BS threads; table of T slots in dynamic
+// LDS, then compaction to (key << 32 | slot) words and an LDS bitonic sort.
+template <int T, int BS, typename V>
+__global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                  const V *__restrict__ aval, const int *__restrict__ brpt,
+                                                  const int *__restrict__ bcol, const V *__restrict__ bval,
+                                                  const int *__restrict__ rows, int nrows,
+                                                  const long long *__restrict__ crpt, int *__restrict__ ccol,
+                                                  V *__restrict__ cval)
+{
+    extern __shared__ unsigned long long smem_u64[];
+    unsigned long long *sk = smem_u64;                               // T/2 sort words
+    V *vt = reinterpret_cast<V *>(smem_u64 + T / 2);                 // T values
+    int *kt = reinterpret_cast<int *>(vt + T);                       // T keys
+    __shared__ int s_pos;
+    constexpr int NU = BS / kWave;
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+
+    for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
+        const int r = rows ? rows[ridx] : ridx;
+        for (int i = tid; i < T; i += BS) {
+            kt[i] = -1;
+            vt[i] = V(0);
+        }
+        if (tid == 0) s_pos = 0;
+        __syncthreads();
+        const FullRow range{brpt};
+        for_each_product<kWave, true, V>(arpt[r], arpt[r + 1], wave, NU, acol, aval, range, [&](int k, V a) {
+            bool fresh;
+            const int h = hash_insert<T>(kt, bcol[k], fresh);
+            atomicAdd(vt + h, a * bval[k]);
+        });
+        __syncthreads();
+        const long long off = crpt[r];
+        const int n = min(static_cast<int>(crpt[r + 1] - off), T / 2);  // rows of this bin have n <= T/2
+        for (int i = tid; i < T; i += BS) {
+            const int key = kt[i];
+            const bool valid = key != -1;
+            const unsigned long long m = __ballot(valid);
+            int base = 0;
+            if (lane_id() == 0 && m) base = atomicAdd(&s_pos, __popcll(m));
+            base = bperm(base, 0);
+            const int pos = base + __popcll(m & ((1ull << lane_id()) - 1ull));
+            if (valid && pos < T / 2) {
+                sk[pos] = (static_cast<unsigned long long>(static_cast<unsigned>(key)) << 32) | static_cast<unsigned>(i);
+            }
+        }
+        int p2 = 1;
+        while (p2 < n) p2 <<= 1;
+        __syncthreads();
+        for (int i = n + tid; i < p2; i += BS) sk[i] = ~0ull;
+        __syncthreads();
+        for (int size = 2; size <= p2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = tid; i < (p2 >> 1); i += BS) {
+                    const int lo = 2 * stride * (i / stride) + (i % stride);
+                    const int hi = lo + stride;
+                    const bool asc = (lo & size) == 0;
+                    const unsigned long long a = sk[lo], b = sk[hi];
+                    if ((a > b) == asc) {
+                        sk[lo] = b;
+                        sk[hi] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int j = tid; j < n; j += BS) {
+            const unsigned long long w = sk[j];
+            ccol[off + j] = static_cast<int>(w >> 32);
+            cval[off + j] = vt[static_cast<unsigned>(w) & (T - 1)];
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kLongBS = 1024;
+
+template <typename V>
+struct DenseWin {
+    static constexpr int W = sizeof(V) == 8 ? 16384 : 32768;  // columns per window: 128 KB of values
+    static constexpr int WORDS = W / 32;
+    static constexpr size_t lds = sizeof(V) * W + sizeof(unsigned) * WORDS;
+};
+
+// Cursor-carrying B range for window [w0, wend): lo is the first entry of
+// the B row not consumed by earlier windows; hi the first entry >= wend,
+// found by galloping from lo.  Tracks the smallest column beyond the window
+// so empty windows can be skipped.
+struct CursorRange {
+    const int *__restrict__ brpt;
+    const int *__restrict__ bcol;
+    int *__restrict__ cursor;
+    int wend;
+    bool first;
+    bool last;  // wend covers all columns
+    int *next_col;
+    __device__ __forceinline__ void operator()(int e, int k, int &bs, int &len) const
+    {
+        const int t = brpt[k + 1];
+        const int lo = first ? brpt[k] : cursor[e];
+        int hi;
+        if (last) {
+            hi = t;
+        } else if (lo >= t || bcol[lo] >= wend) {
+            hi = lo;
+        } else {
+            int prev = lo, step = 1;
+            while (prev + step < t && bcol[prev + step] < wend) {
+                prev += step;
+                step <<= 1;
+            }
+            int l = prev + 1, h = min(prev + step, t);
+            while (l < h) {
+                const int m = (l + h) >> 1;
+                if (bcol[m] < wend)
+                    l = m + 1;
+                else
+                    h = m;
+            }
+            hi = l;
+        }
+        cursor[e] = hi;
+        if (hi < t) *next_col = min(*next_col, bcol[hi]);
+        bs = lo;
+        len = hi - lo;
+    }
+};
+
+// Bin 7: rows above 4096 entries.  Dense LDS accumulator + occupancy bitmap
+// per window of W columns; windows are written out in column order, so the
+// output is sorted without a sort.  cursor[] has one int per A-nonzero.
+template <typename V>
+__global__ __launch_bounds__(kLongBS) void k_num_long(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                      const V *__restrict__ aval, const int *__restrict__ brpt,
+                                                      const int *__restrict__ bcol, const V *__restrict__ bval,
+                                                      const int *__restrict__ rows, int nrows, int ncols,
+                                                      const long long *__restrict__ crpt, int *__restrict__ ccol,
+                                                      V *__restrict__ cval, int *__restrict__ cursor)
+{
+    using D = DenseWin<V>;
+    extern __shared__ unsigned long long smem_u64[];
+    V *acc = reinterpret_cast<V *>(smem_u64);
+    unsigned *bm = reinterpret_cast<unsigned *>(acc + D::W);
+    __shared__ int s_scan[kLongBS / kWave + 1];
+    __shared__ int s_min[kLongBS / kWave];
+    constexpr int NU = kLongBS / kWave;
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+
+    for (int i = tid; i < D::W; i += kLongBS) acc[i] = V(0);
+    for (int i = tid; i < D::WORDS; i += kLongBS) bm[i] = 0u;
+    __syncthreads();
+
+    for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
+        const int r = rows ? rows[ridx] : ridx;
+        const int a_beg = arpt[r], a_end = arpt[r + 1];
+        long long out = crpt[r];
+        const long long row_end = crpt[r + 1];
+        // First window: the one holding the smallest column of the row.
+        int mn = INT_MAX;
+        for (int e = a_beg + tid; e < a_end; e += kLongBS) {
+            const int k = acol[e];
+            if (brpt[k] < brpt[k + 1]) mn = min(mn, bcol[brpt[k]]);
+        }
+        mn = unit_min<kWave>(mn);
+        if (lane_id() == 0) s_min[wave] = mn;
+        __syncthreads();
+        mn = INT_MAX;
+        for (int w = 0; w < NU; w++) mn = min(mn, s_min[w]);
+        __syncthreads();
+        int w0 = mn == INT_MAX ? ncols : (mn / D::W) * D::W;
+        bool first = true;
+        while (w0 < ncols) {
+            const int wend = min(ncols, w0 + D::W);
+            int nxt = INT_MAX;
+            const CursorRange range{brpt, bcol, cursor, wend, first, wend >= ncols, &nxt};
+            for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, [&](int k, V a) {
+                const int c = bcol[k] - w0;
+                atomicAdd(acc + c, a * bval[k]);
+                atomicOr(bm + (c >> 5), 1u << (c & 31));
+            });
+            nxt = unit_min<kWave>(nxt);
+            if (lane_id() == 0) s_min[wave] = nxt;
+            __syncthreads();
+            // Read the window out in column order.
+            for (int wb = 0; wb < D::WORDS; wb += kLongBS) {
+                const int word = wb + tid;
+                const unsigned bits = word < D::WORDS ? bm[word] : 0u;
+                int tot;
+                const int pos = block_excl_scan<kLongBS>(__popc(bits), s_scan, tot);
+                unsigned b = bits;
+                int i = 0;
+                while (b) {
+                    const int bit = __ffs(b) - 1;
+                    b &= b - 1;
+                    const int c = word * 32 + bit;
+                    const long long at = out + pos + i;
+                    if (at < row_end) {
+                        ccol[at] = w0 + c;
+                        cval[at] = acc[c];
+                    }
+                    acc[c] = V(0);
+                    i++;
+                }
+                if (word < D::WORDS) bm[word] = 0u;
+                out += tot;
+            }
+            nxt = INT_MAX;
+            for (int w = 0; w < NU; w++) nxt = min(nxt, s_min[w]);
+            __syncthreads();
+            first = false;
+            w0 = nxt == INT_MAX ? ncols : (nxt / D::W) * D::W;
+        }
+    }
+}
+
+template <int G, int T, int BS, typename V>
+void launch_small(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt, int *ccol,
+                  V *cval, hipStream_t s)
+{
+    constexpr int RPB = BS / G;
+    const int grid = (nrows + RPB - 1) / RPB;
+    k_num_small<G, T, BS, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col,
+                                                static_cast<const V *>(B.val), rows, nrows, crpt, ccol, cval);
+}
+
+template <int T, int BS, typename V>
+constexpr size_t block_lds()
+{
+    return sizeof(unsigned long long) * (T / 2) + sizeof(V) * T + sizeof(int) * T;
+}
+
+template <int T, int BS, typename V>
+void launch_block(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt, int *ccol,
+                  V *cval, hipStream_t s)
+{
+    k_num_block<T, BS, V><<<nrows, BS, block_lds<T, BS, V>(), s>>>(A.rpt, A.col, static_cast<const V *>(A.val), B.rpt,
+                                                                   B.col, static_cast<const V *>(B.val), rows, nrows,
+                                                                   crpt, ccol, cval);
+}
+
+template <typename V>
+void set_attrs()
+{
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_block<8192, 1024, V>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(block_lds<8192, 1024, V>()));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_block<2048, 256, V>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(block_lds<2048, 256, V>()));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_long<V>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(DenseWin<V>::lds));
+}
+
+}  // namespace
+
+void numeric_init_attributes()
+{
+    set_attrs<double>();
+    set_attrs<float>();
+}
+
+template <typename V>
+void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
+                        int *ccol, V *cval, int *cursor, hipStream_t s)
+{
+    if (nrows <= 0) return;
+    switch (bin) {
+        case 0: launch_small<16, 32, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 1: launch_small<32, 64, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 2: launch_small<64, 128, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 3: launch_small<64, 256, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 4: launch_small<64, 512, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 5: launch_block<2048, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 6: launch_block<8192, 1024, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        default:
+            k_num_long<V><<<nrows, kLongBS, DenseWin<V>::lds, s>>>(
+                A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
+                B.N, crpt, ccol, cval, cursor);
+            break;
+    }
+}
+
+template void launch_numeric_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
+                                         double *, int *, hipStream_t);
+template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
+                                        float *, int *, hipStream_t);
+
+}  // namespace opsamd

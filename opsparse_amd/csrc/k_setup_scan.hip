@@ -1,0 +1,343 @@
+// Setup (products per row + symbolic bin histogram), bin scatter, and the
+// row-pointer scan with numeric bin statistics.
+//
+// Reference counterparts: k_compute_flop (src/opsparse/numeric/compute_flop.cu:9-31),
+// k_symbolic_binning / k_symbolic_binning2 (kernels/binning.cuh:11-81),
+// k_numeric_binning / k_numeric_binning2 (binning.cuh:90-174) and the CUB
+// exclusive scan of C.d_rpt (spgemm.cu:49, 113).
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace opsamd {
+
+namespace {
+
+__device__ __forceinline__ int dev_sym_bin(int f)
+{
+    int b = 0;
+#pragma unroll
+    for (int i = 0; i < kNumBins - 1; i++) b += f > kSymBound[i] ? 1 : 0;
+    return b;
+}
+
+__device__ __forceinline__ int dev_num_bin(int n)
+{
+    int b = 0;
+#pragma unroll
+    for (int i = 0; i < kNumBins - 1; i++) b += n > kNumBound[i] ? 1 : 0;
+    return b;
+}
+
+__global__ void k_zero_stats(DevStats *st)
+{
+    unsigned *p = reinterpret_cast<unsigned *>(st);
+    for (unsigned i = threadIdx.x; i < sizeof(DevStats) / 4; i += blockDim.x) p[i] = 0;
+}
+
+// One block per tile of 256 rows.  A-nonzeros of the tile are spread over
+// the threads (so a 40k-entry row costs the same as 40k short rows), each
+// adds its B-row length to its row's LDS counter.
+constexpr int kSetupRows = 256;
+
+__global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                      const int *__restrict__ brpt, int M, int *__restrict__ row_flop,
+                                                      int *__restrict__ row_nnz, DevStats *__restrict__ st)
+{
+    __shared__ int s_rpt[kSetupRows + 1];
+    __shared__ unsigned long long s_flop[kSetupRows];
+    __shared__ int s_hist[kNumBins];
+    __shared__ unsigned long long s_bflop[kNumBins];
+    __shared__ unsigned long long s_red[kSetupRows / kWave];
+    __shared__ int s_max;
+
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * kSetupRows;
+    const int nr = min(kSetupRows, M - r0);
+    if (tid < nr) s_rpt[tid] = arpt[r0 + tid];
+    if (tid == 0) s_rpt[nr] = arpt[r0 + nr];
+    s_flop[tid] = 0;
+    if (tid < kNumBins) {
+        s_hist[tid] = 0;
+        s_bflop[tid] = 0;
+    }
+    if (tid == 0) s_max = 0;
+    __syncthreads();
+
+    const int e_beg = s_rpt[0];
+    const int e_end = s_rpt[nr];
+    for (int e = e_beg + tid; e < e_end; e += kSetupRows) {
+        const int k = acol[e];
+        const int len = brpt[k + 1] - brpt[k];
+        // largest i in [0, nr) with s_rpt[i] <= e
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_rpt[mid] <= e)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        atomicAdd(&s_flop[lo], static_cast<unsigned long long>(len));
+    }
+    __syncthreads();
+
+    unsigned long long f = 0;
+    if (tid < nr) {
+        f = s_flop[tid];
+        const int fi = f > static_cast<unsigned long long>(INT_MAX) ? INT_MAX : static_cast<int>(f);
+        row_flop[r0 + tid] = fi;
+        if (fi == 0) {
+            row_nnz[r0 + tid] = 0;
+        } else {
+            const int b = dev_sym_bin(fi);
+            atomicAdd(&s_hist[b], 1);
+            atomicAdd(&s_bflop[b], f);
+        }
+        atomicMax(&s_max, fi);
+    }
+    const unsigned long long tot = block_sum<kSetupRows>(f, s_red);
+    if (tid < kNumBins && s_hist[tid]) {
+        atomicAdd(&st->sym_size[tid], s_hist[tid]);
+        atomicAdd(&st->sym_flop[tid], s_bflop[tid]);
+    }
+    if (tid == 0) {
+        atomicAdd(&st->total_flop, tot);
+        atomicMax(&st->max_flop, s_max);
+    }
+}
+
+// Scatter row ids into bin lists.  Bin b occupies [sum(size[0..b)), ...).
+// Order inside a bin depends on block scheduling, as in the reference
+// (binning.cuh:75); only the set of rows per bin is fixed.
+__global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ key, int M, int which,
+                                                     DevStats *__restrict__ st, int *__restrict__ bins)
+{
+    __shared__ int s_cnt[kNumBins];
+    __shared__ int s_base[kNumBins];
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * 256 + tid;
+    if (tid < kNumBins) s_cnt[tid] = 0;
+    __syncthreads();
+    int b = -1, local = 0;
+    if (r < M) {
+        const int v = key[r];
+        if (v > 0) {
+            b = which ? dev_num_bin(v) : dev_sym_bin(v);
+            local = atomicAdd(&s_cnt[b], 1);
+        }
+    }
+    __syncthreads();
+    if (tid < kNumBins) {
+        const int *size = which ? st->num_size : st->sym_size;
+        int *fill = which ? st->num_fill : st->sym_fill;
+        int off = 0;
+        for (int i = 0; i < tid; i++) off += size[i];
+        s_base[tid] = s_cnt[tid] ? off + atomicAdd(&fill[tid], s_cnt[tid]) : 0;
+    }
+    __syncthreads();
+    if (b >= 0) bins[s_base[b] + local] = r;
+}
+
+// --- scan: 1024 threads x 4 rows per block ---
+constexpr int kScanBS = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBS * kScanItems;
+
+__global__ __launch_bounds__(kScanBS) void k_scan_reduce(const int *__restrict__ row_nnz,
+                                                         const int *__restrict__ row_flop,
+                                                         const int *__restrict__ arpt, int M,
+                                                         long long *__restrict__ block_sums, DevStats *__restrict__ st)
+{
+    __shared__ unsigned long long s_red[kScanBS / kWave];
+    __shared__ int s_hist[kNumBins];
+    __shared__ unsigned long long s_f[kNumBins], s_n[kNumBins], s_a[kNumBins];
+    __shared__ int s_max;
+    const int tid = threadIdx.x;
+    if (tid < kNumBins) {
+        s_hist[tid] = 0;
+        s_f[tid] = s_n[tid] = s_a[tid] = 0;
+    }
+    if (tid == 0) s_max = 0;
+    __syncthreads();
+    const int r0 = blockIdx.x * kScanTile + tid * kScanItems;
+    unsigned long long sum = 0;
+    int mx = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        const int r = r0 + i;
+        if (r < M) {
+            const int n = row_nnz[r];
+            sum += static_cast<unsigned long long>(n);
+            mx = max(mx, n);
+            if (n > 0) {
+                const int b = dev_num_bin(n);
+                atomicAdd(&s_hist[b], 1);
+                atomicAdd(&s_f[b], static_cast<unsigned long long>(row_flop[r]));
+                atomicAdd(&s_n[b], static_cast<unsigned long long>(n));
+                atomicAdd(&s_a[b], static_cast<unsigned long long>(arpt[r + 1] - arpt[r]));
+            }
+        }
+    }
+    atomicMax(&s_max, mx);
+    const unsigned long long tot = block_sum<kScanBS>(sum, s_red);
+    if (tid == 0) {
+        block_sums[blockIdx.x] = static_cast<long long>(tot);
+        atomicMax(&st->max_nnz, s_max);
+    }
+    if (tid < kNumBins && s_hist[tid]) {
+        atomicAdd(&st->num_size[tid], s_hist[tid]);
+        atomicAdd(&st->num_flop[tid], s_f[tid]);
+        atomicAdd(&st->num_nnz[tid], s_n[tid]);
+        atomicAdd(&st->num_annz[tid], s_a[tid]);
+    }
+}
+
+// Exclusive scan of the block sums, one block; total -> st->total_nnz and
+// block_sums[nb].
+__global__ __launch_bounds__(kScanBS) void k_scan_blocks(long long *__restrict__ block_sums, int nb,
+                                                         DevStats *__restrict__ st)
+{
+    __shared__ long long s_w[kScanBS / kWave + 1];
+    __shared__ long long s_carry;
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wid = tid / kWave;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += kScanBS) {
+        const int i = base + tid;
+        const long long v = i < nb ? block_sums[i] : 0;
+        long long incl = v;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const long long t = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl += t;
+        }
+        if (lane == kWave - 1) s_w[wid] = incl;
+        __syncthreads();
+        if (tid == 0) {
+            long long run = 0;
+            for (int w = 0; w < kScanBS / kWave; w++) {
+                const long long t = s_w[w];
+                s_w[w] = run;
+                run += t;
+            }
+            s_w[kScanBS / kWave] = run;
+        }
+        __syncthreads();
+        const long long carry = s_carry;
+        if (i < nb) block_sums[i] = carry + s_w[wid] + incl - v;
+        __syncthreads();
+        if (tid == 0) s_carry = carry + s_w[kScanBS / kWave];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        block_sums[nb] = s_carry;
+        st->total_nnz = static_cast<unsigned long long>(s_carry);
+    }
+}
+
+__global__ __launch_bounds__(kScanBS) void k_scan_write(const int *__restrict__ row_nnz, int M,
+                                                        const long long *__restrict__ block_sums, int nb,
+                                                        long long *__restrict__ crpt64, int *__restrict__ crpt32)
+{
+    __shared__ long long s_w[kScanBS / kWave + 1];
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wid = tid / kWave;
+    const int r0 = blockIdx.x * kScanTile + tid * kScanItems;
+    int v[kScanItems];
+    long long loc = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = r0 + i < M ? row_nnz[r0 + i] : 0;
+        loc += v[i];
+    }
+    long long incl = loc;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const long long t = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += t;
+    }
+    if (lane == kWave - 1) s_w[wid] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        long long run = 0;
+        for (int w = 0; w < kScanBS / kWave; w++) {
+            const long long t = s_w[w];
+            s_w[w] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    const long long total = block_sums[nb];
+    const bool fits = total <= static_cast<long long>(INT_MAX);
+    long long run = block_sums[blockIdx.x] + s_w[wid] + incl - loc;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        const int r = r0 + i;
+        if (r < M) {
+            crpt64[r] = run;
+            if (crpt32 && fits) crpt32[r] = static_cast<int>(run);
+        }
+        run += v[i];
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        crpt64[M] = total;
+        if (crpt32 && fits) crpt32[M] = static_cast<int>(total);
+    }
+}
+
+__global__ void k_narrow(const long long *__restrict__ src, int n, int *__restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = static_cast<int>(src[i]);
+}
+
+__global__ void k_rownnz(const long long *__restrict__ crpt, int M, int *__restrict__ row_nnz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) row_nnz[i] = static_cast<int>(crpt[i + 1] - crpt[i]);
+}
+
+}  // namespace
+
+void launch_rownnz_from_rpt(const long long *crpt, int M, int *row_nnz, hipStream_t s)
+{
+    if (M <= 0) return;
+    k_rownnz<<<(M + 255) / 256, 256, 0, s>>>(crpt, M, row_nnz);
+}
+
+void launch_zero_stats(DevStats *st, hipStream_t s) { k_zero_stats<<<1, 256, 0, s>>>(st); }
+
+void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, hipStream_t s)
+{
+    if (A.M == 0) return;
+    const int grid = (A.M + kSetupRows - 1) / kSetupRows;
+    k_setup<<<grid, kSetupRows, 0, s>>>(A.rpt, A.col, B.rpt, A.M, row_flop, row_nnz, st);
+}
+
+void launch_bin_scatter(const int *key, int M, int which, DevStats *st, int *bins, hipStream_t s)
+{
+    if (M == 0) return;
+    k_bin_scatter<<<(M + 255) / 256, 256, 0, s>>>(key, M, which, st, bins);
+}
+
+int scan_blocks(int M) { return (M + kScanTile - 1) / kScanTile; }
+
+void launch_scan(const int *row_nnz, const int *row_flop, const int *arpt, int M, long long *block_sums,
+                 long long *crpt64, int *crpt32, DevStats *st, hipStream_t s)
+{
+    const int nb = scan_blocks(M);
+    if (nb > 0) k_scan_reduce<<<nb, kScanBS, 0, s>>>(row_nnz, row_flop, arpt, M, block_sums, st);
+    k_scan_blocks<<<1, kScanBS, 0, s>>>(block_sums, nb, st);
+    k_scan_write<<<nb > 0 ? nb : 1, kScanBS, 0, s>>>(row_nnz, M, block_sums, nb, crpt64, crpt32);
+}
+
+void launch_narrow_rpt(const long long *crpt64, int M, int *crpt32, hipStream_t s)
+{
+    const int n = M + 1;
+    k_narrow<<<(n + 255) / 256, 256, 0, s>>>(crpt64, n, crpt32);
+}
+
+}  // namespace opsamd

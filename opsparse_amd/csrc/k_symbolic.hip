@@ -1,0 +1,180 @@
+// Symbolic phase: nnz of every row of C.
+//
+// Reference counterparts: k_symbolic_shared_hash_pwarp / _tb / _large_ /
+// _max_..._with_fail / k_symbolic_global_hash_tb
+// (include/opsparse/core_algo/spgemm/kernels/symbolic.cuh:9-325) and their
+// dispatch h_symbolic (src/opsparse/core_algo/spgemm/spgemm_phases.cu:52-121).
+//
+// Bins 0-6: LDS hash table of T int32 keys per row with T > row products,
+// so a row never overflows its table and there is no fail/retry pass (the
+// reference's bin 7 probes, fails and recomputes, symbolic.cuh:192-261).
+// New keys are counted per lane and summed once per row, not with an LDS
+// atomic per insert.
+// Bin 7 (more than 16384 products): an LDS bitmap over up to 2^20 columns
+// per window; a set bit is a column of C, counted when its atomicOr flips it.
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace opsamd {
+
+namespace {
+
+// G lanes per row (G <= 64: BS/G rows per block; G == BS > 64: one row per
+// block, G/64 waves share the table).
+template <int G, int T, int BS>
+__global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                 const int *__restrict__ brpt, const int *__restrict__ bcol,
+                                                 const int *__restrict__ rows, int nrows, int *__restrict__ row_nnz)
+{
+    constexpr int U = G < kWave ? G : kWave;
+    constexpr int RPB = BS / G;
+    constexpr int NU = G / U;
+    static_assert(BS % G == 0 && (T & (T - 1)) == 0, "bad config");
+    __shared__ int tab[RPB * T];
+    __shared__ int s_cnt[RPB];
+
+    const int tid = threadIdx.x;
+    const int slot = tid / G;
+    const int unit = (tid % G) / U;
+    for (int i = tid; i < RPB * T; i += BS) tab[i] = -1;
+    if (NU > 1 && tid < RPB) s_cnt[tid] = 0;
+    __syncthreads();
+
+    const int ridx = blockIdx.x * RPB + slot;
+    int cnt = 0;
+    int r = 0;
+    if (ridx < nrows) {
+        r = rows ? rows[ridx] : ridx;
+        int *t = tab + slot * T;
+        const FullRow range{brpt};
+        for_each_product<U, false, double>(arpt[r], arpt[r + 1], unit, NU, acol, static_cast<const double *>(nullptr),
+                                           range, [&](int k, double) {
+                                               bool fresh;
+                                               hash_insert<T>(t, bcol[k], fresh);
+                                               cnt += fresh ? 1 : 0;
+                                           });
+    }
+    if constexpr (NU == 1) {
+        cnt = unit_sum<U>(cnt);
+        if ((tid % G) == 0 && ridx < nrows) row_nnz[r] = cnt;
+    } else {
+        cnt = unit_sum<kWave>(cnt);
+        if (lane_id() == 0) atomicAdd(&s_cnt[0], cnt);
+        __syncthreads();
+        if (tid == 0 && ridx < nrows) row_nnz[r] = s_cnt[0];
+    }
+}
+
+constexpr int kLongBS = 1024;
+constexpr int kBitmapWords = 32768;  // 128 KB: 2^20 columns per window
+constexpr int kBitmapCols = kBitmapWords * 32;
+
+// B range of an A-nonzero restricted to columns [w0, wend).  For a single
+// window covering all columns this is the whole row.
+struct WindowRange {
+    const int *__restrict__ brpt;
+    const int *__restrict__ bcol;
+    int w0, wend;
+    bool whole;
+    __device__ __forceinline__ void operator()(int /*e*/, int k, int &bs, int &len) const
+    {
+        const int s = brpt[k], t = brpt[k + 1];
+        if (whole) {
+            bs = s;
+            len = t - s;
+            return;
+        }
+        int lo = s, hi = t;
+        while (lo < hi) {
+            const int m = (lo + hi) >> 1;
+            if (bcol[m] < w0)
+                lo = m + 1;
+            else
+                hi = m;
+        }
+        const int first = lo;
+        hi = t;
+        while (lo < hi) {
+            const int m = (lo + hi) >> 1;
+            if (bcol[m] < wend)
+                lo = m + 1;
+            else
+                hi = m;
+        }
+        bs = first;
+        len = lo - first;
+    }
+};
+
+__global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                      const int *__restrict__ brpt, const int *__restrict__ bcol,
+                                                      const int *__restrict__ rows, int nrows, int ncols,
+                                                      int *__restrict__ row_nnz)
+{
+    extern __shared__ unsigned bm[];
+    __shared__ int s_red[kLongBS / kWave];
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    constexpr int NW = kLongBS / kWave;
+    for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
+        const int r = rows ? rows[ridx] : ridx;
+        const int a_beg = arpt[r], a_end = arpt[r + 1];
+        int cnt = 0;
+        for (int w0 = 0; w0 < ncols; w0 += kBitmapCols) {
+            const int wend = min(ncols, w0 + kBitmapCols);
+            const int nwords = (wend - w0 + 31) >> 5;
+            for (int i = tid; i < nwords; i += kLongBS) bm[i] = 0u;
+            __syncthreads();
+            const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols};
+            for_each_product<kWave, false, double>(a_beg, a_end, wave, NW, acol, static_cast<const double *>(nullptr),
+                                                   range, [&](int k, double) {
+                                                       const int c = bcol[k] - w0;
+                                                       const unsigned bit = 1u << (c & 31);
+                                                       const unsigned old = atomicOr(&bm[c >> 5], bit);
+                                                       cnt += (old & bit) ? 0 : 1;
+                                                   });
+            __syncthreads();
+        }
+        const int tot = block_sum<kLongBS>(cnt, s_red);
+        if (tid == 0) row_nnz[r] = tot;
+    }
+}
+
+template <int G, int T, int BS>
+void launch_hash(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, hipStream_t s)
+{
+    constexpr int RPB = BS / G;
+    const int grid = (nrows + RPB - 1) / RPB;
+    k_sym_hash<G, T, BS><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+}
+
+}  // namespace
+
+size_t symbolic_long_lds() { return sizeof(unsigned) * kBitmapWords; }
+
+void symbolic_init_attributes()
+{
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_sym_long), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(symbolic_long_lds()));
+}
+
+void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz,
+                         hipStream_t s)
+{
+    if (nrows <= 0) return;
+    switch (bin) {
+        case 0: launch_hash<16, 32, 256>(A, B, rows, nrows, row_nnz, s); break;
+        case 1: launch_hash<32, 64, 256>(A, B, rows, nrows, row_nnz, s); break;
+        case 2: launch_hash<64, 128, 256>(A, B, rows, nrows, row_nnz, s); break;
+        case 3: launch_hash<64, 512, 256>(A, B, rows, nrows, row_nnz, s); break;
+        case 4: launch_hash<64, 1024, 256>(A, B, rows, nrows, row_nnz, s); break;
+        case 5: launch_hash<256, 4096, 256>(A, B, rows, nrows, row_nnz, s); break;
+        case 6: launch_hash<1024, 16384, 1024>(A, B, rows, nrows, row_nnz, s); break;
+        default:
+            k_sym_long<<<nrows, kLongBS, symbolic_long_lds(), s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, B.N,
+                                                                  row_nnz);
+            break;
+    }
+}
+
+}  // namespace opsamd

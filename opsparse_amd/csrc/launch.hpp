@@ -1,0 +1,108 @@
+// Host-side declarations of the SpGEMM kernel launchers and the row-bin
+// configuration shared by the kernels and the pipeline driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+namespace opsamd {
+
+constexpr int kNumBins = 8;
+
+// Symbolic bins by row products f (inclusive upper bounds); rows with f == 0
+// are not binned (their nnz is 0).  Bin 7 is the long-row path.  The
+// reference bins by the same quantity with bounds {26, 426, ..., 10240, inf}
+// sized for 32-lane warps and 48 KB of shared memory (binning.cuh:21, 51);
+// these are sized for 64-lane units and 160 KB of LDS.
+//   bin:  0    1    2    3     4      5      6       7
+//   G:    16   32   64   64    64     256    1024    1024 (lanes per row)
+//   T:    32   64   128  512   1024   4096   16384   bitmap, 2^20 columns per window
+constexpr int kSymBound[kNumBins] = {16, 32, 64, 256, 1024, 4096, 16384, INT_MAX};
+
+// Numeric bins by exact row nnz n (inclusive upper bounds); rows with n == 0
+// are not binned.  The reference uses {16, 128, ..., 4095, inf}
+// (binning.cuh:105, 143).  Hash tables hold >= 2n slots.
+//   bin:  0    1    2    3     4     5      6      7
+//   G:    16   32   64   64    64    256    1024   1024
+//   T:    32   64   128  256   512   2048   8192   dense column windows
+//   sort: unit bitonic in registers (bins 0-4), LDS bitonic (5-6), none (7)
+constexpr int kNumBound[kNumBins] = {16, 32, 64, 128, 256, 1024, 4096, INT_MAX};
+
+// Device-side counters of one call (zeroed by k_zero_stats).
+struct DevStats {
+    unsigned long long total_flop;
+    unsigned long long total_nnz;
+    int max_flop;
+    int max_nnz;
+    int sym_size[kNumBins];
+    int sym_fill[kNumBins];
+    int num_size[kNumBins];
+    int num_fill[kNumBins];
+    unsigned long long sym_flop[kNumBins];
+    unsigned long long num_flop[kNumBins];
+    unsigned long long num_nnz[kNumBins];
+    unsigned long long num_annz[kNumBins];
+};
+
+struct CsrIn {
+    const int *rpt;
+    const int *col;
+    const void *val;
+    int M, N;
+};
+
+// --- setup / binning / scan (k_setup_scan.hip) ---
+void launch_zero_stats(DevStats *st, hipStream_t s);
+// row_flop[M] (int32, saturated), per-bin counts; rows with f == 0 get row_nnz = 0.
+void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, hipStream_t s);
+// Scatters row ids into bins (which = 0: symbolic by row_flop, 1: numeric by row_nnz).
+void launch_bin_scatter(const int *key, int M, int which, DevStats *st, int *bins, hipStream_t s);
+// Exclusive scan of row_nnz into crpt64 (and crpt32 when nnz(C) fits int32
+// and crpt32 != nullptr); numeric bin statistics.  block_sums needs
+// scan_blocks(M) + 1 int64 entries.
+int scan_blocks(int M);
+void launch_scan(const int *row_nnz, const int *row_flop, const int *arpt, int M, long long *block_sums,
+                 long long *crpt64, int *crpt32, DevStats *st, hipStream_t s);
+void launch_narrow_rpt(const long long *crpt64, int M, int *crpt32, hipStream_t s);
+
+// --- symbolic (k_symbolic.hip) ---
+// Computes row_nnz for the rows of one bin.  rows == nullptr means rows 0..nrows-1.
+void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz,
+                         hipStream_t s);
+// Bytes of dynamic LDS the long-row symbolic kernel asks for.
+size_t symbolic_long_lds();
+
+// --- numeric (k_numeric.hip) ---
+template <typename V>
+void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
+                        int *ccol, V *cval, int *cursor, hipStream_t s);
+// One-time kernel attribute setup (dynamic LDS limits).
+void numeric_init_attributes();
+void symbolic_init_attributes();
+
+// --- backward (k_backward.hip) ---
+template <typename V>
+void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
+                         const int *ccol, const V *dc, V *da, V *db, hipStream_t s);
+void backward_init_attributes();
+void launch_zero_values(void *p, long long n, bool f32, hipStream_t s);
+// row_nnz[i] = crpt[i+1] - crpt[i] (numeric binning key for the backward).
+void launch_rownnz_from_rpt(const long long *crpt, int M, int *row_nnz, hipStream_t s);
+
+inline int sym_bin_of(long long f)
+{
+    for (int b = 0; b < kNumBins; b++)
+        if (f <= kSymBound[b]) return b;
+    return kNumBins - 1;
+}
+
+inline int num_bin_of(long long n)
+{
+    for (int b = 0; b < kNumBins; b++)
+        if (n <= kNumBound[b]) return b;
+    return kNumBins - 1;
+}
+
+}  // namespace opsamd

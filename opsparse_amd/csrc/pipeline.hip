@@ -1,0 +1,772 @@
+// Host driver of the SpGEMM pipeline and the C ABI (include/opsparse_amd.h).
+//
+// Phase order follows the reference driver (src/opsparse/core_algo/spgemm/spgemm.cu:16-126):
+// setup -> symbolic binning -> symbolic -> scan + numeric binning ->
+// allocate C -> numeric.  Differences, all MI355X-side:
+//  * the workspace (streams, events, buffers) persists across calls instead
+//    of being created and destroyed per call (meta.cu:16-65);
+//  * two host synchronisations per call (after setup, after the scan) instead
+//    of 4-7 device-wide syncs;
+//  * per-bin kernels run on a pool of 8 HIP streams forked from and joined
+//    back to the caller's stream with events, so the call is stream-ordered;
+//  * C's row offsets are computed in 64 bits, so products above 2^31 nonzeros
+//    are supported by the 64-bit entry points.
+#include <hip/hip_runtime.h>
+#include <time.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "launch.hpp"
+#include "opsparse_amd.h"
+
+using namespace opsamd;
+
+namespace {
+
+double now_s()
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC_RAW, &ts);
+    return static_cast<double>(ts.tv_sec) + static_cast<double>(ts.tv_nsec) * 1e-9;
+}
+
+struct HipFail : std::runtime_error {
+    int status;
+    HipFail(int st, const std::string &m) : std::runtime_error(m), status(st) {}
+};
+
+void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        const int st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? OPS_OOM : OPS_HIP_ERROR;
+        (void)hipGetLastError();
+        throw HipFail(st, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+#define CK(x) check((x), #x)
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n)
+    {
+        if (n <= cap && p) return;
+        if (p) CK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(n + n / 8, 256);
+        CK(hipMalloc(&p, want * sizeof(T)));
+        cap = want;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+bool g_attrs_done = false;
+
+}  // namespace
+
+struct ops_workspace {
+    int device = 0;
+    hipStream_t pool[kNumBins] = {};
+    hipStream_t cap = nullptr;  // origin stream for graph capture
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[kNumBins] = {};
+    hipEvent_t ev_k0[kNumBins] = {}, ev_k1[kNumBins] = {};
+    DevStats *d_stats = nullptr;
+    DevStats *h_stats = nullptr;  // pinned
+    DevBuf<int> row_flop, row_nnz, bins, cursor, crpt32_tmp;
+    DevBuf<long long> block_sums, crpt64;
+    std::string err;
+};
+
+// Everything the numeric phase needs, produced by the symbolic phase.
+struct PlanCore {
+    int M = 0, N = 0;
+    long long nnz = 0;
+    long long *crpt64 = nullptr;  // M+1, device
+    int *bins = nullptr;          // numeric bin lists (nullptr: identity over one bin)
+    int *cursor = nullptr;        // one int per A-nonzero, long-row bin only
+    int num_size[kNumBins] = {};
+    int num_off[kNumBins] = {};
+    DevStats stats{};
+};
+
+struct ops_plan {
+    ops_workspace *ws = nullptr;
+    PlanCore core;
+    DevBuf<long long> crpt64;
+    DevBuf<int> bins, cursor;
+    int annz = 0;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_src = nullptr;
+    // arguments the captured graph was recorded with
+    const void *g_args[8] = {};
+    int g_dt = -1;
+};
+
+namespace {
+
+CsrIn as_in(const ops_csr *m) { return CsrIn{m->rpt, m->col, m->val, m->M, m->N}; }
+
+void validate(const ops_csr *A, const ops_csr *B)
+{
+    if (!A || !B) throw HipFail(OPS_INVALID, "null matrix descriptor");
+    if (A->M < 0 || A->N < 0 || B->M < 0 || B->N < 0 || A->nnz < 0 || B->nnz < 0)
+        throw HipFail(OPS_INVALID, "negative dimension");
+    if (A->N != B->M) throw HipFail(OPS_NONSQUARE, "A.N != B.M");
+    if (A->M > 0 && (!A->rpt || (A->nnz > 0 && !A->col))) throw HipFail(OPS_INVALID, "A device arrays missing");
+    if (B->M > 0 && (!B->rpt || (B->nnz > 0 && !B->col))) throw HipFail(OPS_INVALID, "B device arrays missing");
+}
+
+void init_attrs()
+{
+    if (g_attrs_done) return;
+    symbolic_init_attributes();
+    numeric_init_attributes();
+    backward_init_attributes();
+    g_attrs_done = true;
+}
+
+// Runs `launch(bin, stream)` for each non-empty bin on the pool streams,
+// forked from and joined back to `s`.  ms[] receives event-timed kernel
+// durations when timing is on.
+template <typename F>
+void dispatch_bins(ops_workspace *ws, hipStream_t s, const int *size, bool timing, F &&launch)
+{
+    int used = 0;
+    for (int b = 0; b < kNumBins; b++) used += size[b] > 0 ? 1 : 0;
+    if (used == 0) return;
+    CK(hipEventRecord(ws->ev_fork, s));
+    // Largest bins first so the long kernels start earliest.
+    const int order[kNumBins] = {7, 6, 5, 4, 3, 2, 1, 0};
+    for (int oi = 0; oi < kNumBins; oi++) {
+        const int b = order[oi];
+        if (size[b] <= 0) continue;
+        hipStream_t ps = ws->pool[b];
+        CK(hipStreamWaitEvent(ps, ws->ev_fork, 0));
+        if (timing) CK(hipEventRecord(ws->ev_k0[b], ps));
+        launch(b, ps);
+        CK(hipGetLastError());
+        if (timing) CK(hipEventRecord(ws->ev_k1[b], ps));
+        CK(hipEventRecord(ws->ev_join[b], ps));
+        CK(hipStreamWaitEvent(s, ws->ev_join[b], 0));
+    }
+}
+
+void read_kernel_ms(ops_workspace *ws, const int *size, double *ms)
+{
+    for (int b = 0; b < kNumBins; b++) {
+        ms[b] = 0.0;
+        if (size[b] <= 0) continue;
+        float f = 0.f;
+        CK(hipEventSynchronize(ws->ev_k1[b]));
+        CK(hipEventElapsedTime(&f, ws->ev_k0[b], ws->ev_k1[b]));
+        ms[b] = f;
+    }
+}
+
+// Setup, symbolic and scan phases.  crpt64_dst: where C's 64-bit row
+// offsets go; crpt32_dst (optional): int32 copy, written only if nnz fits.
+void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore &pc, long long *crpt64_dst,
+                    int *crpt32_dst, int *bins_dst, hipStream_t s, ops_timings *t)
+{
+    const bool sep = t && t->measure_separate;
+    const bool ev = t && t->kernel_events;
+    const int M = A.M;
+    pc.M = M;
+    pc.N = B.N;
+    pc.crpt64 = crpt64_dst;
+    double t0 = now_s();
+
+    ws->row_flop.ensure(static_cast<size_t>(M) + 1);
+    ws->row_nnz.ensure(static_cast<size_t>(M) + 1);
+    ws->block_sums.ensure(static_cast<size_t>(scan_blocks(M)) + 2);
+    launch_zero_stats(ws->d_stats, s);
+    launch_setup(A, B, ws->row_flop.p, ws->row_nnz.p, ws->d_stats, s);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(ws->h_stats, ws->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    DevStats st = *ws->h_stats;
+    if (t) {
+        t->setup = now_s() - t0;
+        t->total_flop = static_cast<int64_t>(st.total_flop);
+        t->max_row_flop = st.max_flop;
+        for (int b = 0; b < kNumBins; b++) {
+            t->sym_rows[b] = st.sym_size[b];
+            t->sym_flop[b] = static_cast<int64_t>(st.sym_flop[b]);
+        }
+    }
+
+    // Symbolic binning: identity when every row is in one bin.
+    t0 = now_s();
+    int sym_off[kNumBins];
+    const int *sym_rows = nullptr;
+    {
+        int run = 0, nonempty = 0, only = -1;
+        for (int b = 0; b < kNumBins; b++) {
+            sym_off[b] = run;
+            run += st.sym_size[b];
+            if (st.sym_size[b]) {
+                nonempty++;
+                only = b;
+            }
+        }
+        if (!(nonempty == 1 && st.sym_size[only] == M) && nonempty > 0) {
+            launch_bin_scatter(ws->row_flop.p, M, 0, ws->d_stats, bins_dst, s);
+            CK(hipGetLastError());
+            sym_rows = bins_dst;
+        }
+    }
+    if (sep) {
+        CK(hipStreamSynchronize(s));
+        t->symbolic_binning = now_s() - t0;
+    }
+
+    t0 = now_s();
+    dispatch_bins(ws, s, st.sym_size, ev, [&](int b, hipStream_t ps) {
+        launch_symbolic_bin(b, A, B, sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b], ws->row_nnz.p, ps);
+    });
+    if (sep) {
+        CK(hipStreamSynchronize(s));
+        t->symbolic = now_s() - t0;
+    }
+    if (ev) read_kernel_ms(ws, st.sym_size, t->sym_ms);
+
+    t0 = now_s();
+    launch_scan(ws->row_nnz.p, ws->row_flop.p, A.rpt, M, ws->block_sums.p, crpt64_dst, crpt32_dst, ws->d_stats, s);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(ws->h_stats, ws->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    st = *ws->h_stats;
+    pc.stats = st;
+    pc.nnz = static_cast<long long>(st.total_nnz);
+    if (t) {
+        t->prefix = now_s() - t0;
+        t->nnz_c = pc.nnz;
+        t->max_row_nnz = st.max_nnz;
+        for (int b = 0; b < kNumBins; b++) {
+            t->num_rows[b] = st.num_size[b];
+            t->num_flop[b] = static_cast<int64_t>(st.num_flop[b]);
+            t->num_nnz[b] = static_cast<int64_t>(st.num_nnz[b]);
+            t->num_arows_nnz[b] = static_cast<int64_t>(st.num_annz[b]);
+        }
+    }
+
+    // Numeric binning (the rows' bins are final once nnz is known).
+    t0 = now_s();
+    int run = 0, nonempty = 0, only = -1;
+    for (int b = 0; b < kNumBins; b++) {
+        pc.num_size[b] = st.num_size[b];
+        pc.num_off[b] = run;
+        run += st.num_size[b];
+        if (st.num_size[b]) {
+            nonempty++;
+            only = b;
+        }
+    }
+    pc.bins = nullptr;
+    if (!(nonempty == 1 && st.num_size[only] == M) && nonempty > 0) {
+        launch_bin_scatter(ws->row_nnz.p, M, 1, ws->d_stats, bins_dst, s);
+        CK(hipGetLastError());
+        pc.bins = bins_dst;
+    }
+    if (sep) {
+        CK(hipStreamSynchronize(s));
+        t->numeric_binning = now_s() - t0;
+    }
+}
+
+template <typename V>
+void numeric_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, const PlanCore &pc, int *ccol, V *cval,
+                   hipStream_t s, ops_timings *t)
+{
+    const bool ev = t && t->kernel_events;
+    dispatch_bins(ws, s, pc.num_size, ev, [&](int b, hipStream_t ps) {
+        launch_numeric_bin<V>(b, A, B, pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b], pc.crpt64, ccol,
+                              cval, pc.cursor, ps);
+    });
+}
+
+template <typename F>
+int guarded(ops_workspace *ws, F &&f)
+{
+    try {
+        return f();
+    } catch (const HipFail &e) {
+        if (ws) ws->err = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &e) {
+        if (ws) ws->err = e.what();
+        return OPS_OOM;
+    } catch (const std::exception &e) {
+        if (ws) ws->err = e.what();
+        return OPS_EXCEPTION;
+    } catch (...) {
+        if (ws) ws->err = "unknown exception";
+        return OPS_EXCEPTION;
+    }
+}
+
+// Column-index span of A's descriptor: the highest A-nonzero index + 1,
+// which is what per-A-nonzero scratch must cover (descriptors may be row
+// slices whose rpt does not start at 0).
+long long a_index_span(const ops_csr *A, hipStream_t s)
+{
+    if (A->M == 0) return 0;
+    int last = 0;
+    CK(hipMemcpyAsync(&last, A->rpt + A->M, sizeof(int), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    return last;
+}
+
+template <typename V>
+int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *C32, ops_csr64 *C64, hipStream_t s,
+                ops_timings *t)
+{
+    const double t_start = now_s();
+    validate(A, B);
+    if (!C32 && !C64) throw HipFail(OPS_INVALID, "null output descriptor");
+    init_attrs();
+    const CsrIn a = as_in(A), b = as_in(B);
+    const int M = A->M;
+    PlanCore pc;
+    ws->bins.ensure(static_cast<size_t>(M) + 1);
+    // On any failure below, C's partial arrays are released.
+    struct Cleanup {
+        ops_csr *c32;
+        ops_csr64 *c64;
+        bool armed = true;
+        ~Cleanup()
+        {
+            if (!armed) return;
+            if (c32) ops_csr_free(c32);
+            if (c64) ops_csr64_free(c64);
+        }
+    } cleanup{C32, C64};
+
+    double t0 = now_s();
+    long long *crpt64_dst;
+    int *crpt32_dst = nullptr;
+    if (C64) {
+        *C64 = ops_csr64{};
+        C64->M = M;
+        C64->N = B->N;
+        CK(hipMalloc(&C64->rpt, sizeof(int64_t) * (static_cast<size_t>(M) + 1)));
+        crpt64_dst = reinterpret_cast<long long *>(C64->rpt);
+    } else {
+        *C32 = ops_csr{};
+        C32->M = M;
+        C32->N = B->N;
+        CK(hipMalloc(&C32->rpt, sizeof(int32_t) * (static_cast<size_t>(M) + 1)));
+        ws->crpt64.ensure(static_cast<size_t>(M) + 1);
+        crpt64_dst = ws->crpt64.p;
+        crpt32_dst = C32->rpt;
+    }
+    const double t_alloc_rpt = now_s() - t0;
+
+    symbolic_phase(ws, a, b, pc, crpt64_dst, crpt32_dst, ws->bins.p, s, t);
+    if (t) t->setup += t_alloc_rpt;
+
+    if (C32 && pc.nnz > static_cast<long long>(INT32_MAX))
+        throw HipFail(OPS_INT32_OVERFLOW, "nnz(C) exceeds INT32_MAX; use the 64-bit entry point");
+
+    t0 = now_s();
+    const size_t nz = static_cast<size_t>(std::max<long long>(pc.nnz, 1));
+    int *ccol = nullptr;
+    V *cval = nullptr;
+    CK(hipMalloc(&ccol, nz * sizeof(int)));
+    CK(hipMalloc(&cval, nz * sizeof(V)));
+    if (C64) {
+        C64->nnz = pc.nnz;
+        C64->col = ccol;
+        C64->val = cval;
+    } else {
+        C32->nnz = static_cast<int32_t>(pc.nnz);
+        C32->col = ccol;
+        C32->val = cval;
+    }
+    if (pc.num_size[kNumBins - 1] > 0) {
+        ws->cursor.ensure(static_cast<size_t>(a_index_span(A, s)) + 1);
+        pc.cursor = ws->cursor.p;
+    }
+    if (t) t->allocate = now_s() - t0;
+
+    t0 = now_s();
+    numeric_phase<V>(ws, a, b, pc, ccol, cval, s, t);
+    CK(hipStreamSynchronize(s));
+    if (t) {
+        t->numeric = now_s() - t0;
+        if (t->kernel_events) read_kernel_ms(ws, pc.num_size, t->num_ms);
+        t->cleanup = 0.0;
+        t->total = now_s() - t_start;
+    }
+    cleanup.armed = false;
+    return OPS_OK;
+}
+
+template <typename V>
+void backward_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const int64_t *crpt64, const int32_t *ccol,
+                   const V *dc, V *da, V *db, hipStream_t s)
+{
+    validate(A, B);
+    init_attrs();
+    const int M = A->M;
+    if (M > 0 && !crpt64) throw HipFail(OPS_INVALID, "C row offsets missing");
+    const CsrIn a = as_in(A), b = as_in(B);
+    ws->row_nnz.ensure(static_cast<size_t>(M) + 1);
+    ws->bins.ensure(static_cast<size_t>(M) + 1);
+    ws->block_sums.ensure(static_cast<size_t>(scan_blocks(M)) + 2);
+    const long long *crpt = reinterpret_cast<const long long *>(crpt64);
+    // dA entries of rows without products and all of dB start at zero.
+    launch_zero_values(da, a_index_span(A, s), sizeof(V) == 4, s);
+    launch_zero_values(db, a_index_span(B, s), sizeof(V) == 4, s);
+    launch_zero_stats(ws->d_stats, s);
+    launch_rownnz_from_rpt(crpt, M, ws->row_nnz.p, s);
+    CK(hipGetLastError());
+    if (M == 0) return;
+    // Numeric bin sizes of C's rows (the scan's own output is unused here).
+    ws->crpt64.ensure(static_cast<size_t>(M) + 1);
+    launch_scan(ws->row_nnz.p, ws->row_nnz.p, A->rpt, M, ws->block_sums.p, ws->crpt64.p, nullptr, ws->d_stats, s);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(ws->h_stats, ws->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    const DevStats st = *ws->h_stats;
+    int off[kNumBins];
+    int run = 0, nonempty = 0, only = -1;
+    for (int bb = 0; bb < kNumBins; bb++) {
+        off[bb] = run;
+        run += st.num_size[bb];
+        if (st.num_size[bb]) {
+            nonempty++;
+            only = bb;
+        }
+    }
+    const int *rows = nullptr;
+    if (!(nonempty == 1 && st.num_size[only] == M) && nonempty > 0) {
+        launch_bin_scatter(ws->row_nnz.p, M, 1, ws->d_stats, ws->bins.p, s);
+        CK(hipGetLastError());
+        rows = ws->bins.p;
+    }
+    dispatch_bins(ws, s, st.num_size, false, [&](int bin, hipStream_t ps) {
+        launch_backward_bin<V>(bin, a, b, rows ? rows + off[bin] : nullptr, st.num_size[bin], crpt, ccol, dc, da, db,
+                               ps);
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ops_version(void) { return "opsparse_amd 0.1 (gfx950)"; }
+
+int ops_workspace_create(ops_workspace **out)
+{
+    if (!out) return OPS_INVALID;
+    *out = nullptr;
+    ops_workspace *ws = new (std::nothrow) ops_workspace();
+    if (!ws) return OPS_OOM;
+    const int st = guarded(ws, [&]() {
+        CK(hipGetDevice(&ws->device));
+        for (int b = 0; b < kNumBins; b++) {
+            CK(hipStreamCreateWithFlags(&ws->pool[b], hipStreamNonBlocking));
+            CK(hipEventCreateWithFlags(&ws->ev_join[b], hipEventDisableTiming));
+            CK(hipEventCreate(&ws->ev_k0[b]));
+            CK(hipEventCreate(&ws->ev_k1[b]));
+        }
+        CK(hipEventCreateWithFlags(&ws->ev_fork, hipEventDisableTiming));
+        CK(hipStreamCreateWithFlags(&ws->cap, hipStreamNonBlocking));
+        CK(hipMalloc(&ws->d_stats, sizeof(DevStats)));
+        CK(hipHostMalloc(&ws->h_stats, sizeof(DevStats), hipHostMallocDefault));
+        return OPS_OK;
+    });
+    if (st != OPS_OK) {
+        ops_workspace_destroy(ws);
+        return st;
+    }
+    *out = ws;
+    return OPS_OK;
+}
+
+int ops_workspace_destroy(ops_workspace *ws)
+{
+    if (!ws) return OPS_INVALID;
+    for (int b = 0; b < kNumBins; b++) {
+        if (ws->pool[b]) (void)hipStreamDestroy(ws->pool[b]);
+        if (ws->ev_join[b]) (void)hipEventDestroy(ws->ev_join[b]);
+        if (ws->ev_k0[b]) (void)hipEventDestroy(ws->ev_k0[b]);
+        if (ws->ev_k1[b]) (void)hipEventDestroy(ws->ev_k1[b]);
+    }
+    if (ws->ev_fork) (void)hipEventDestroy(ws->ev_fork);
+    if (ws->cap) (void)hipStreamDestroy(ws->cap);
+    if (ws->d_stats) (void)hipFree(ws->d_stats);
+    if (ws->h_stats) (void)hipHostFree(ws->h_stats);
+    ws->row_flop.release();
+    ws->row_nnz.release();
+    ws->bins.release();
+    ws->cursor.release();
+    ws->crpt32_tmp.release();
+    ws->block_sums.release();
+    ws->crpt64.release();
+    delete ws;
+    return OPS_OK;
+}
+
+const char *ops_workspace_last_error(const ops_workspace *ws) { return ws ? ws->err.c_str() : "null workspace"; }
+
+int ops_csr_free(ops_csr *C)
+{
+    if (!C) return OPS_INVALID;
+    if (C->rpt) (void)hipFree(C->rpt);
+    if (C->col) (void)hipFree(C->col);
+    if (C->val) (void)hipFree(C->val);
+    *C = ops_csr{};
+    return OPS_OK;
+}
+
+int ops_csr64_free(ops_csr64 *C)
+{
+    if (!C) return OPS_INVALID;
+    if (C->rpt) (void)hipFree(C->rpt);
+    if (C->col) (void)hipFree(C->col);
+    if (C->val) (void)hipFree(C->val);
+    *C = ops_csr64{};
+    return OPS_OK;
+}
+
+int ops_spgemm_f64(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *C, void *stream, ops_timings *t)
+{
+    if (!ws) return OPS_INVALID;
+    return guarded(ws, [&]() {
+        return spgemm_impl<double>(ws, A, B, C, nullptr, static_cast<hipStream_t>(stream), t);
+    });
+}
+
+int ops_spgemm_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *C, void *stream, ops_timings *t)
+{
+    if (!ws) return OPS_INVALID;
+    return guarded(ws, [&]() {
+        return spgemm_impl<float>(ws, A, B, C, nullptr, static_cast<hipStream_t>(stream), t);
+    });
+}
+
+int ops_spgemm64_f64(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr64 *C, void *stream, ops_timings *t)
+{
+    if (!ws) return OPS_INVALID;
+    return guarded(ws, [&]() {
+        return spgemm_impl<double>(ws, A, B, nullptr, C, static_cast<hipStream_t>(stream), t);
+    });
+}
+
+int ops_spgemm64_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr64 *C, void *stream, ops_timings *t)
+{
+    if (!ws) return OPS_INVALID;
+    return guarded(ws, [&]() {
+        return spgemm_impl<float>(ws, A, B, nullptr, C, static_cast<hipStream_t>(stream), t);
+    });
+}
+
+int ops_spgemm_safe_f64(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const int32_t *a_rpt_host,
+                        const int32_t *a_col_host, const int32_t *b_rpt_host, ops_csr *C, void *stream,
+                        ops_timings *t, int64_t *total_flop_out, int64_t *nnz_c_out, size_t *free_before,
+                        size_t *free_after, size_t *total_bytes)
+{
+    if (total_flop_out) *total_flop_out = 0;
+    if (nnz_c_out) *nnz_c_out = -1;
+    if (!A || !B || !C) return OPS_INVALID;
+    // Pre-flight on the host, before any GPU work (spgemm.cu:154-177).
+    if (A->N != B->M) return OPS_NONSQUARE;
+    if (!a_rpt_host || !a_col_host || !b_rpt_host) return OPS_INVALID;
+    const int64_t flop = ops_compute_flop_host(A->M, a_rpt_host, a_col_host, b_rpt_host, nullptr);
+    if (total_flop_out) *total_flop_out = flop;
+    if (flop > static_cast<int64_t>(INT32_MAX)) return OPS_INT32_OVERFLOW;
+    if (!ws) return OPS_INVALID;
+    size_t fb = 0, tb = 0;
+    if (hipMemGetInfo(&fb, &tb) != hipSuccess) fb = tb = 0;
+    (void)hipGetLastError();
+    if (free_before) *free_before = fb;
+    if (total_bytes) *total_bytes = tb;
+    const int st = guarded(ws, [&]() {
+        return spgemm_impl<double>(ws, A, B, C, nullptr, static_cast<hipStream_t>(stream), t);
+    });
+    size_t fa = 0;
+    if (hipMemGetInfo(&fa, &tb) != hipSuccess) fa = 0;
+    (void)hipGetLastError();
+    if (free_after) *free_after = fa;
+    if (st == OPS_OK && nnz_c_out) *nnz_c_out = C->nnz;
+    return st;
+}
+
+int ops_compute_flop_device(ops_workspace *ws, const ops_csr *A, const ops_csr *B, int32_t *row_flop,
+                            int64_t *total_out, void *stream)
+{
+    if (!ws || !row_flop) return OPS_INVALID;
+    return guarded(ws, [&]() {
+        validate(A, B);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        ws->row_nnz.ensure(static_cast<size_t>(A->M) + 1);
+        launch_zero_stats(ws->d_stats, s);
+        launch_setup(as_in(A), as_in(B), row_flop, ws->row_nnz.p, ws->d_stats, s);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(ws->h_stats, ws->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        if (total_out) *total_out = static_cast<int64_t>(ws->h_stats->total_flop);
+        return OPS_OK;
+    });
+}
+
+// ---- plans ----
+
+int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_plan **plan, int64_t *nnz_c,
+                    void *stream)
+{
+    if (!ws || !plan) return OPS_INVALID;
+    *plan = nullptr;
+    ops_plan *p = new (std::nothrow) ops_plan();
+    if (!p) return OPS_OOM;
+    p->ws = ws;
+    const int st = guarded(ws, [&]() {
+        validate(A, B);
+        init_attrs();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const int M = A->M;
+        p->crpt64.ensure(static_cast<size_t>(M) + 1);
+        p->bins.ensure(static_cast<size_t>(M) + 1);
+        symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr);
+        if (p->core.num_size[kNumBins - 1] > 0) {
+            p->cursor.ensure(static_cast<size_t>(a_index_span(A, s)) + 1);
+            p->core.cursor = p->cursor.p;
+        }
+        CK(hipStreamSynchronize(s));
+        if (nnz_c) *nnz_c = p->core.nnz;
+        return OPS_OK;
+    });
+    if (st != OPS_OK) {
+        ops_plan_destroy(p);
+        return st;
+    }
+    *plan = p;
+    return OPS_OK;
+}
+
+int ops_plan_destroy(ops_plan *plan)
+{
+    if (!plan) return OPS_INVALID;
+    if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
+    if (plan->graph_src) (void)hipGraphDestroy(plan->graph_src);
+    plan->crpt64.release();
+    plan->bins.release();
+    plan->cursor.release();
+    delete plan;
+    return OPS_OK;
+}
+
+int ops_plan_rowptr64(const ops_plan *plan, int64_t *rpt, void *stream)
+{
+    if (!plan || !rpt) return OPS_INVALID;
+    return guarded(plan->ws, [&]() {
+        CK(hipMemcpyAsync(rpt, plan->core.crpt64, sizeof(int64_t) * (static_cast<size_t>(plan->core.M) + 1),
+                          hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+        return OPS_OK;
+    });
+}
+
+int ops_plan_rowptr32(const ops_plan *plan, int32_t *rpt, void *stream)
+{
+    if (!plan || !rpt) return OPS_INVALID;
+    if (plan->core.nnz > static_cast<long long>(INT32_MAX)) return OPS_INT32_OVERFLOW;
+    return guarded(plan->ws, [&]() {
+        launch_narrow_rpt(plan->core.crpt64, plan->core.M, rpt, static_cast<hipStream_t>(stream));
+        CK(hipGetLastError());
+        return OPS_OK;
+    });
+}
+
+int ops_plan_numeric(ops_plan *plan, ops_dtype dt, const ops_csr *A, const ops_csr *B, int32_t *ccol, void *cval,
+                     void *stream)
+{
+    if (!plan || !A || !B || (plan->core.nnz > 0 && (!ccol || !cval))) return OPS_INVALID;
+    if (dt != OPS_F64 && dt != OPS_F32) return OPS_INVALID;
+    if (A->M != plan->core.M || B->N != plan->core.N) return OPS_INVALID;
+    return guarded(plan->ws, [&]() {
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (dt == OPS_F64)
+            numeric_phase<double>(plan->ws, as_in(A), as_in(B), plan->core, ccol, static_cast<double *>(cval), s,
+                                  nullptr);
+        else
+            numeric_phase<float>(plan->ws, as_in(A), as_in(B), plan->core, ccol, static_cast<float *>(cval), s,
+                                 nullptr);
+        return OPS_OK;
+    });
+}
+
+int ops_spgemm_backward(ops_workspace *ws, ops_dtype dt, const ops_csr *A, const ops_csr *B, const int64_t *crpt64,
+                        const int32_t *ccol, const void *dcval, void *da, void *db, void *stream)
+{
+    if (!ws || !A || !B) return OPS_INVALID;
+    if (dt != OPS_F64 && dt != OPS_F32) return OPS_INVALID;
+    if ((A->nnz > 0 && !da) || (B->nnz > 0 && !db)) return OPS_INVALID;
+    return guarded(ws, [&]() {
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (dt == OPS_F64)
+            backward_impl<double>(ws, A, B, crpt64, ccol, static_cast<const double *>(dcval), static_cast<double *>(da),
+                                  static_cast<double *>(db), s);
+        else
+            backward_impl<float>(ws, A, B, crpt64, ccol, static_cast<const float *>(dcval), static_cast<float *>(da),
+                                 static_cast<float *>(db), s);
+        return OPS_OK;
+    });
+}
+
+int ops_plan_numeric_graph(ops_plan *plan, ops_dtype dt, const ops_csr *A, const ops_csr *B, int32_t *ccol,
+                           void *cval, void *stream)
+{
+    if (!plan || !A || !B || (plan->core.nnz > 0 && (!ccol || !cval))) return OPS_INVALID;
+    if (dt != OPS_F64 && dt != OPS_F32) return OPS_INVALID;
+    if (A->M != plan->core.M || B->N != plan->core.N) return OPS_INVALID;
+    return guarded(plan->ws, [&]() {
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const void *args[8] = {A->rpt, A->col, A->val, B->rpt, B->col, B->val, ccol, cval};
+        bool same = plan->graph && plan->g_dt == static_cast<int>(dt);
+        for (int i = 0; i < 8 && same; i++) same = plan->g_args[i] == args[i];
+        if (!same) {
+            if (plan->graph) CK(hipGraphExecDestroy(plan->graph));
+            if (plan->graph_src) CK(hipGraphDestroy(plan->graph_src));
+            plan->graph = nullptr;
+            plan->graph_src = nullptr;
+            // Capture on a private stream so the caller's stream may be the null stream.
+            hipStream_t cs = plan->ws->cap;
+            CK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            try {
+                if (dt == OPS_F64)
+                    numeric_phase<double>(plan->ws, as_in(A), as_in(B), plan->core, ccol, static_cast<double *>(cval),
+                                          cs, nullptr);
+                else
+                    numeric_phase<float>(plan->ws, as_in(A), as_in(B), plan->core, ccol, static_cast<float *>(cval),
+                                         cs, nullptr);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(cs, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            CK(hipStreamEndCapture(cs, &plan->graph_src));
+            CK(hipGraphInstantiate(&plan->graph, plan->graph_src, nullptr, nullptr, 0));
+            for (int i = 0; i < 8; i++) plan->g_args[i] = args[i];
+            plan->g_dt = static_cast<int>(dt);
+        }
+        CK(hipGraphLaunch(plan->graph, s));
+        return OPS_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+"""PyTorch-ROCm host interface over the C ABI (include/opsparse_amd.h).
+
+Device memory and streams come from torch; every multiply runs the
+library's HIP kernels through ctypes.  There is no CPU fallback: without
+the built library or a GPU these functions raise.
+
+Mirrors the reference's operator surface (include/opsparse/core_algo/spgemm/spgemm.hpp:12-22):
+``spgemm`` is opsparse_spgemm (the library allocates C), ``spgemm_safe``
+is opsparse_spgemm_safe (status instead of exceptions), plus the
+structure-reuse ``Plan``, the batched call and the differentiable
+``SpGEMMFunction`` that have no reference counterpart.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import (OPS_F32, OPS_F64, OPS_OK, OpsCsr, OpsCsr64, OpsError, OpsTimings, check, lib)
+from .gen import HostCSR
+
+_HIP = None
+
+
+def _hip():
+    """The HIP runtime already loaded by torch (same SONAME as the library's)."""
+    global _HIP
+    if _HIP is None:
+        _HIP = C.CDLL("libamdhip64.so.7")
+        _HIP.hipMemcpy.restype = C.c_int
+        _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return _HIP
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("opsparse_amd needs a HIP GPU; none is visible")
+
+
+def _stream_ptr(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float64:
+        return OPS_F64
+    if dtype == torch.float32:
+        return OPS_F32
+    raise TypeError(f"unsupported value dtype {dtype}")
+
+
+@dataclass
+class DeviceCSR:
+    """CSR on the GPU as torch tensors: rpt int32 (M+1), col int32, val f32/f64."""
+    M: int
+    N: int
+    rpt: torch.Tensor
+    col: torch.Tensor
+    val: torch.Tensor
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.numel())
+
+    @staticmethod
+    def from_host(h: HostCSR, dtype=torch.float64, device="cuda") -> "DeviceCSR":
+        return DeviceCSR(h.M, h.N, torch.from_numpy(np.ascontiguousarray(h.rpt)).to(device),
+                         torch.from_numpy(np.ascontiguousarray(h.col)).to(device),
+                         torch.from_numpy(np.ascontiguousarray(h.val)).to(device=device, dtype=dtype))
+
+    def desc(self) -> OpsCsr:
+        d = OpsCsr()
+        d.M, d.N, d.nnz = self.M, self.N, self.nnz
+        d.rpt, d.col, d.val = self.rpt.data_ptr(), self.col.data_ptr(), self.val.data_ptr()
+        return d
+
+    def with_values(self, val: torch.Tensor) -> "DeviceCSR":
+        return DeviceCSR(self.M, self.N, self.rpt, self.col, val)
+
+
+class Workspace:
+    """Persistent per-device workspace (stream pool, events, buffers)."""
+
+    def __init__(self):
+        _require_gpu()
+        self.ptr = C.c_void_p()
+        check(lib().ops_workspace_create(C.byref(self.ptr)))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib().ops_workspace_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+_default_ws = None
+
+
+def default_workspace() -> Workspace:
+    global _default_ws
+    if _default_ws is None:
+        _default_ws = Workspace()
+    return _default_ws
+
+
+class LibCSR:
+    """C allocated by the library (opsparse_spgemm semantics); freed with
+    ops_csr_free / ops_csr64_free when this object dies."""
+
+    def __init__(self, desc, wide: bool, dtype: torch.dtype):
+        self.d = desc
+        self.wide = wide
+        self.dtype = dtype
+
+    @property
+    def M(self):
+        return int(self.d.M)
+
+    @property
+    def N(self):
+        return int(self.d.N)
+
+    @property
+    def nnz(self):
+        return int(self.d.nnz)
+
+    def to_host(self):
+        """(rpt int64, col int32, val) numpy copies."""
+        hip = _hip()
+        M, nnz = self.M, self.nnz
+        ridx = np.int64 if self.wide else np.int32
+        rpt = np.empty(M + 1, dtype=ridx)
+        col = np.empty(nnz, dtype=np.int32)
+        val = np.empty(nnz, dtype=np.float64 if self.dtype == torch.float64 else np.float32)
+        torch.cuda.synchronize()
+        for dst, src, n in ((rpt, self.d.rpt, rpt.nbytes), (col, self.d.col, col.nbytes), (val, self.d.val, val.nbytes)):
+            if n:
+                st = hip.hipMemcpy(dst.ctypes.data, src, n, 2)
+                if st != 0:
+                    raise RuntimeError(f"hipMemcpy failed: {st}")
+        return rpt.astype(np.int64), col, val
+
+    def free(self):
+        if self.d is not None:
+            (lib().ops_csr64_free if self.wide else lib().ops_csr_free)(C.byref(self.d))
+            self.d = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def spgemm(A: DeviceCSR, B: DeviceCSR, ws: Workspace | None = None, timings: OpsTimings | None = None,
+           wide: bool = False, stream=None) -> LibCSR:
+    """C = A*B through ops_spgemm{,64}_{f64,f32}: the library allocates C."""
+    _require_gpu()
+    if A.val.dtype != B.val.dtype:
+        raise TypeError("A and B must have the same value dtype")
+    ws = ws or default_workspace()
+    a, b = A.desc(), B.desc()
+    f64 = A.val.dtype == torch.float64
+    t = C.byref(timings) if timings is not None else None
+    if wide:
+        c = OpsCsr64()
+        fn = lib().ops_spgemm64_f64 if f64 else lib().ops_spgemm64_f32
+    else:
+        c = OpsCsr()
+        fn = lib().ops_spgemm_f64 if f64 else lib().ops_spgemm_f32
+    check(fn(ws.ptr, C.byref(a), C.byref(b), C.byref(c), _stream_ptr(stream), t), ws.ptr)
+    return LibCSR(c, wide, A.val.dtype)
+
+
+def spgemm_safe(A: DeviceCSR, B: DeviceCSR, A_host: HostCSR, B_host: HostCSR, ws: Workspace | None = None):
+    """opsparse_spgemm_safe: returns (status, LibCSR or None, info dict)."""
+    ws = ws or default_workspace()
+    a, b, c = A.desc(), B.desc(), OpsCsr()
+    flop, cnnz = C.c_int64(), C.c_int64()
+    fb, fa, tot = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    st = lib().ops_spgemm_safe_f64(ws.ptr, C.byref(a), C.byref(b), A_host.rpt.ctypes.data, A_host.col.ctypes.data,
+                                   B_host.rpt.ctypes.data, C.byref(c), _stream_ptr(None), None, C.byref(flop),
+                                   C.byref(cnnz), C.byref(fb), C.byref(fa), C.byref(tot))
+    info = dict(total_flop=flop.value, c_nnz=cnnz.value, free_before=fb.value, free_after=fa.value,
+                total_bytes=tot.value)
+    return st, (LibCSR(c, False, torch.float64) if st == OPS_OK else None), info
+
+
+class Plan:
+    """Structure of C = A*B for fixed patterns: C's row offsets and numeric
+    bins, computed once; the numeric phase can then run many times (also
+    as a replayed hipGraph)."""
+
+    def __init__(self, A: DeviceCSR, B: DeviceCSR, ws: Workspace | None = None, stream=None):
+        _require_gpu()
+        self.ws = ws or default_workspace()
+        self.M, self.N = A.M, B.N
+        self.ptr = C.c_void_p()
+        nnz = C.c_int64()
+        a, b = A.desc(), B.desc()
+        check(lib().ops_plan_create(self.ws.ptr, C.byref(a), C.byref(b), C.byref(self.ptr), C.byref(nnz),
+                                    _stream_ptr(stream)), self.ws.ptr)
+        self.nnz = int(nnz.value)
+        self._rpt64 = None
+
+    def rowptr64(self, stream=None) -> torch.Tensor:
+        if self._rpt64 is None:
+            t = torch.empty(self.M + 1, dtype=torch.int64, device="cuda")
+            check(lib().ops_plan_rowptr64(self.ptr, t.data_ptr(), _stream_ptr(stream)), self.ws.ptr)
+            self._rpt64 = t
+        return self._rpt64
+
+    def rowptr32(self, stream=None) -> torch.Tensor:
+        t = torch.empty(self.M + 1, dtype=torch.int32, device="cuda")
+        check(lib().ops_plan_rowptr32(self.ptr, t.data_ptr(), _stream_ptr(stream)), self.ws.ptr)
+        return t
+
+    def numeric(self, A: DeviceCSR, B: DeviceCSR, col: torch.Tensor | None = None, val: torch.Tensor | None = None,
+                graph: bool = False, stream=None):
+        dt = A.val.dtype
+        if col is None:
+            col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device="cuda")
+        if val is None:
+            val = torch.empty(max(self.nnz, 1), dtype=dt, device="cuda")
+        a, b = A.desc(), B.desc()
+        fn = lib().ops_plan_numeric_graph if graph else lib().ops_plan_numeric
+        check(fn(self.ptr, dtype_code(dt), C.byref(a), C.byref(b), col.data_ptr(), val.data_ptr(),
+                 _stream_ptr(stream)), self.ws.ptr)
+        return col[:self.nnz], val[:self.nnz]
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib().ops_plan_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def backward(A: DeviceCSR, B: DeviceCSR, crpt64: torch.Tensor, ccol: torch.Tensor, dc: torch.Tensor,
+             ws: Workspace | None = None, stream=None):
+    """(dA values on pattern(A), dB values on pattern(B)) for L with dL/dC = dc."""
+    ws = ws or default_workspace()
+    dt = A.val.dtype
+    da = torch.empty(max(A.nnz, 1), dtype=dt, device="cuda")
+    db = torch.empty(max(B.nnz, 1), dtype=dt, device="cuda")
+    a, b = A.desc(), B.desc()
+    dc = dc.contiguous()
+    check(lib().ops_spgemm_backward(ws.ptr, dtype_code(dt), C.byref(a), C.byref(b), crpt64.data_ptr(),
+                                    ccol.data_ptr(), dc.data_ptr(), da.data_ptr(), db.data_ptr(),
+                                    _stream_ptr(stream)), ws.ptr)
+    return da[:A.nnz], db[:B.nnz]
+
+
+class SpGEMMFunction(torch.autograd.Function):
+    """C_values = spgemm(A, B) on fixed patterns, differentiable in the
+    values of A and B.  Backward is the masked pair
+    dA = (dC B^T) on pattern(A), dB = (A^T dC) on pattern(B), computed by
+    the library's HIP backward kernels (C's structure is recomputed from
+    the plan, no hash tables are kept)."""
+
+    @staticmethod
+    def forward(ctx, a_val, b_val, A: DeviceCSR, B: DeviceCSR, plan: Plan):
+        Aw, Bw = A.with_values(a_val.contiguous()), B.with_values(b_val.contiguous())
+        col, val = plan.numeric(Aw, Bw)
+        ctx.save_for_backward(a_val, b_val)
+        ctx.A, ctx.B, ctx.plan, ctx.col = A, B, plan, col
+        ctx.mark_non_differentiable(col)
+        return val, col
+
+    @staticmethod
+    def backward(ctx, dval, dcol):
+        a_val, b_val = ctx.saved_tensors
+        Aw, Bw = ctx.A.with_values(a_val.contiguous()), ctx.B.with_values(b_val.contiguous())
+        da, db = backward(Aw, Bw, ctx.plan.rowptr64(), ctx.col, dval.contiguous())
+        return da, db, None, None, None
+
+
+def spgemm_autograd(a_val: torch.Tensor, b_val: torch.Tensor, A: DeviceCSR, B: DeviceCSR, plan: Plan | None = None):
+    """Differentiable C = A*B; returns (plan, C values, C columns)."""
+    plan = plan or Plan(A, B)
+    val, col = SpGEMMFunction.apply(a_val, b_val, A, B, plan)
+    return plan, val, col
+
+
+__all__ = ["DeviceCSR", "Workspace", "LibCSR", "Plan", "spgemm", "spgemm_safe", "backward", "SpGEMMFunction",
+           "spgemm_autograd", "default_workspace", "OpsTimings", "OpsError", "dtype_code"]

@@ -1,0 +1,93 @@
+"""C ABI checks that need no GPU: exports, host pre-checks, host flop count."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+
+import oracle
+import opsparse_amd
+import opsparse_amd.gen as G
+from opsparse_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"^[A-Za-z_][\w \*]*?\b(ops_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 25, names
+    h = C.CDLL(opsparse_amd.LIB_PATH)
+    for n in names:
+        assert hasattr(h, n), f"{n} declared in include/ but not exported"
+    assert set(names) == set(_lib.SIGNATURES), "ctypes table out of sync with the header"
+
+
+def test_cpp_dropin_symbols_exported():
+    out = subprocess.run(["nm", "-DC", "--defined-only", opsparse_amd.LIB_PATH], capture_output=True, text=True).stdout
+    for sym in ("opsparse::opsparse_spgemm(", "opsparse::opsparse_spgemm_reg(", "opsparse::opsparse_spgemm_safe(",
+                "opsparse::CSR::H2D()", "opsparse::CSR::D2H()", "opsparse::CSR::construct(",
+                "opsparse::compute_flop(", "opsparse::SpgemmTimings::print(", "opsparse::to_cstr(",
+                "opsparse::io::read_mm_banner("):
+        assert sym in out, sym
+
+
+def test_version():
+    assert b"gfx950" in _lib.lib().ops_version()
+
+
+def _desc(M, N, nnz):
+    d = _lib.OpsCsr()
+    d.M, d.N, d.nnz = M, N, nnz
+    return d
+
+
+def test_safe_precheck_nonsquare_without_gpu():
+    a, b, c = _desc(4, 5, 0), _desc(6, 4, 0), _lib.OpsCsr()
+    rpt = np.zeros(8, dtype=np.int32)
+    st = _lib.lib().ops_spgemm_safe_f64(None, C.byref(a), C.byref(b), rpt.ctypes.data, rpt.ctypes.data,
+                                        rpt.ctypes.data, C.byref(c), None, None, None, None, None, None, None)
+    assert st == _lib.OPS_NONSQUARE
+
+
+def test_safe_precheck_int32_overflow_without_gpu():
+    # A: 3 rows with one nonzero each pointing at B rows whose lengths sum past INT32_MAX.
+    a, b, c = _desc(3, 3, 3), _desc(3, 3, 0), _lib.OpsCsr()
+    arpt = np.array([0, 1, 2, 3], dtype=np.int32)
+    acol = np.array([0, 1, 2], dtype=np.int32)
+    brpt = np.array([0, 2**30, 2**31 - 2, 2**31 - 1], dtype=np.int64).astype(np.int32)
+    brpt[2] = 2**31 - 2
+    flop = C.c_int64()
+    st = _lib.lib().ops_spgemm_safe_f64(None, C.byref(a), C.byref(b), arpt.ctypes.data, acol.ctypes.data,
+                                        brpt.ctypes.data, C.byref(c), None, None, C.byref(flop), None, None, None,
+                                        None)
+    # row lengths 2^30, 2^31-2-2^30, 1 -> total 2^31 - 1: fits; make it overflow by repeating a long row
+    assert st in (_lib.OPS_OK, _lib.OPS_INVALID, _lib.OPS_INT32_OVERFLOW)
+    acol2 = np.array([0, 0, 0], dtype=np.int32)  # three times the 2^30 row
+    st = _lib.lib().ops_spgemm_safe_f64(None, C.byref(a), C.byref(b), arpt.ctypes.data, acol2.ctypes.data,
+                                        brpt.ctypes.data, C.byref(c), None, None, C.byref(flop), None, None, None,
+                                        None)
+    assert st == _lib.OPS_INT32_OVERFLOW and flop.value == 3 * 2**30
+
+
+def test_invalid_arguments_without_gpu():
+    a, b, c = _desc(4, 4, 0), _desc(4, 4, 0), _lib.OpsCsr()
+    assert _lib.lib().ops_spgemm_f64(None, C.byref(a), C.byref(b), C.byref(c), None, None) == _lib.OPS_INVALID
+    assert _lib.lib().ops_plan_create(None, C.byref(a), C.byref(b), None, None, None) == _lib.OPS_INVALID
+    assert _lib.lib().ops_workspace_create(None) == _lib.OPS_INVALID
+    assert _lib.lib().ops_csr_free(None) == _lib.OPS_INVALID
+
+
+def test_host_flop_matches_oracle():
+    A = G.rmat(11, 8, 1)
+    assert G.compute_flop_host(A, A) == oracle.compute_flop(A, A)

@@ -1,0 +1,141 @@
+"""GPU parity of the HIP SpGEMM against the CPU oracle.
+
+Structure (row_ptr, sorted columns) must be bit-exact; values must be
+within max |d| / sum_k |a_ik b_kj| <= 1e-12 (FP64) or 1e-5 (FP32), the
+summation-order-independent error bound.  Every call goes through the C ABI
+of libopsparse_amd.so.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import opsparse_amd.gen as G
+from opsparse_amd.spgemm import DeviceCSR, Plan, spgemm
+from helpers import TOL, check_product, csr_from_dense, csr_from_rows, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def run(A, B, dtype=np.float64, wide=False):
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    dA = DeviceCSR.from_host(A, tdt)
+    dB = DeviceCSR.from_host(B, tdt)
+    C = spgemm(dA, dB, wide=wide)
+    rpt, col, val = C.to_host()
+    return check_product(rpt, col, val, A, B, dtype)
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def test_tiny_dense(gpu):
+    rng = np.random.default_rng(11)
+    D1 = rng.uniform(-1, 1, (7, 5)) * (rng.random((7, 5)) < 0.6)
+    D2 = rng.uniform(-1, 1, (5, 9)) * (rng.random((5, 9)) < 0.6)
+    A, B = csr_from_dense(D1), csr_from_dense(D2)
+    run(A, B)
+    # cross-check the dense triple loop
+    ref = oracle.spgemm(A, B)
+    dense = np.zeros((7, 9))
+    for i in range(7):
+        for p in range(ref[0][i], ref[0][i + 1]):
+            dense[i, ref[1][p]] = ref[2][p]
+    assert np.allclose(dense, D1 @ D2, atol=1e-14)
+
+
+def test_identity_and_diagonal(gpu):
+    rng = np.random.default_rng(1)
+    n = 300
+    B = random_csr(n, n, rng.integers(0, 40, n), rng)
+    I = G.HostCSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n))
+    run(I, B)
+    run(B, I)
+    Dg = G.HostCSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), rng.uniform(1, 2, n))
+    run(Dg, B)
+
+
+def test_empty_rows_and_empty_product(gpu):
+    rng = np.random.default_rng(2)
+    n = 200
+    lens = rng.integers(0, 6, n)
+    lens[::3] = 0
+    A = random_csr(n, n, lens, rng)
+    B = random_csr(n, n, np.where(np.arange(n) % 2 == 0, 0, 5), rng)
+    run(A, B)
+    Z = G.HostCSR(n, n, np.zeros(n + 1, dtype=np.int32), np.zeros(0, np.int32), np.zeros(0))
+    run(Z, B)
+    run(A, Z)
+
+
+def test_uniform_4096_fp64_fp32(gpu):
+    A = G.uniform_rows(4096, 4096, 8, 0)
+    log("uniform fp64")
+    e = run(A, A, np.float64)
+    assert e["max_norm"] <= TOL[np.float64]
+    log("uniform fp32")
+    run(A, A, np.float32)
+
+
+def test_stencil_small(gpu):
+    A = G.stencil27(24, 1)
+    run(A, A)
+
+
+def test_every_bin_rmat(gpu):
+    A = G.rmat(13, 16, 7)
+    t = G.compute_flop_host(A, A)
+    log(f"rmat13 flop {t}")
+    run(A, A)
+    run(A, A, np.float32)
+
+
+def test_long_rows_windows(gpu):
+    """Rows with > 16384 products (symbolic long path) and > 4096 entries
+    (numeric dense windows over several 16384-column windows)."""
+    rng = np.random.default_rng(3)
+    N = 50000
+    A = random_csr(40, N, np.r_[rng.integers(150, 260, 20), rng.integers(1, 4, 20)], rng)
+    B = random_csr(N, N, rng.integers(60, 140, N), rng)
+    run(A, B)
+    run(A, B, np.float32)
+
+
+def test_symbolic_multi_window(gpu):
+    """More than 2^20 columns: the symbolic bitmap runs in several windows."""
+    rng = np.random.default_rng(4)
+    N = (1 << 21) + 777
+    K = 400
+    A = random_csr(8, K, rng.integers(100, 200, 8), rng)
+    B = random_csr(K, N, rng.integers(150, 300, K), rng)
+    run(A, B)
+
+
+def test_wide_rowptr(gpu):
+    A = G.rmat(11, 8, 5)
+    run(A, A, wide=True)
+
+
+def test_plan_and_graph(gpu):
+    A = G.rmat(12, 12, 9)
+    dA = DeviceCSR.from_host(A, torch.float64)
+    p = Plan(dA, dA)
+    ref = oracle.spgemm(A, A)
+    rpt = p.rowptr64().cpu().numpy()
+    col, val = p.numeric(dA, dA)
+    oracle.compare(rpt, col.cpu().numpy(), val.cpu().numpy(), ref, TOL[np.float64])
+    col_g = torch.empty(p.nnz, dtype=torch.int32, device="cuda")
+    val_g = torch.empty(p.nnz, dtype=torch.float64, device="cuda")
+    for rep in range(3):
+        val_g.fill_(7.0)
+        p.numeric(dA, dA, col_g, val_g, graph=True)
+        torch.cuda.synchronize()
+        oracle.compare(rpt, col_g.cpu().numpy(), val_g.cpu().numpy(), ref, TOL[np.float64])
+        # new values, same pattern, replay of the same graph
+    A2 = G.HostCSR(A.M, A.N, A.rpt, A.col, A.val * 2.0)
+    dA2 = DeviceCSR.from_host(A2, torch.float64)
+    dA.val.copy_(dA2.val)
+    p.numeric(dA, dA, col_g, val_g, graph=True)
+    torch.cuda.synchronize()
+    oracle.compare(rpt, col_g.cpu().numpy(), val_g.cpu().numpy(), oracle.spgemm(A2, A2), TOL[np.float64])
