@@ -84,6 +84,7 @@ typedef struct ops_timings {
      * (ms are 0 unless kernel_events was set) */
     int64_t sym_rows[OPS_NUM_BINS];
     int64_t sym_flop[OPS_NUM_BINS];
+    int64_t sym_arows_nnz[OPS_NUM_BINS]; /* nnz of A over the rows of the bin */
     double sym_ms[OPS_NUM_BINS];
     int64_t num_rows[OPS_NUM_BINS];
     int64_t num_flop[OPS_NUM_BINS];

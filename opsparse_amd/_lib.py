@@ -49,7 +49,7 @@ class OpsTimings(C.Structure):
                 ("allocate", C.c_double), ("numeric", C.c_double), ("cleanup", C.c_double), ("total", C.c_double),
                 ("total_flop", C.c_int64), ("nnz_c", C.c_int64), ("max_row_flop", C.c_int32),
                 ("max_row_nnz", C.c_int32),
-                ("sym_rows", C.c_int64 * NUM_BINS), ("sym_flop", C.c_int64 * NUM_BINS),
+                ("sym_rows", C.c_int64 * NUM_BINS), ("sym_flop", C.c_int64 * NUM_BINS), ("sym_arows_nnz", C.c_int64 * NUM_BINS),
                 ("sym_ms", C.c_double * NUM_BINS),
                 ("num_rows", C.c_int64 * NUM_BINS), ("num_flop", C.c_int64 * NUM_BINS),
                 ("num_nnz", C.c_int64 * NUM_BINS), ("num_arows_nnz", C.c_int64 * NUM_BINS),

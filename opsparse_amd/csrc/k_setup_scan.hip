@@ -47,6 +47,7 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
     __shared__ unsigned long long s_flop[kSetupRows];
     __shared__ int s_hist[kNumBins];
     __shared__ unsigned long long s_bflop[kNumBins];
+    __shared__ unsigned long long s_bannz[kNumBins];
     __shared__ unsigned long long s_red[kSetupRows / kWave];
     __shared__ int s_max;
 
@@ -59,6 +60,7 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
     if (tid < kNumBins) {
         s_hist[tid] = 0;
         s_bflop[tid] = 0;
+        s_bannz[tid] = 0;
     }
     if (tid == 0) s_max = 0;
     __syncthreads();
@@ -92,6 +94,7 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
             const int b = dev_sym_bin(fi);
             atomicAdd(&s_hist[b], 1);
             atomicAdd(&s_bflop[b], f);
+            atomicAdd(&s_bannz[b], static_cast<unsigned long long>(s_rpt[tid + 1] - s_rpt[tid]));
         }
         atomicMax(&s_max, fi);
     }
@@ -99,6 +102,7 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
     if (tid < kNumBins && s_hist[tid]) {
         atomicAdd(&st->sym_size[tid], s_hist[tid]);
         atomicAdd(&st->sym_flop[tid], s_bflop[tid]);
+        atomicAdd(&st->sym_annz[tid], s_bannz[tid]);
     }
     if (tid == 0) {
         atomicAdd(&st->total_flop, tot);

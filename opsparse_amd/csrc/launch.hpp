@@ -41,6 +41,7 @@ struct DevStats {
     int num_size[kNumBins];
     int num_fill[kNumBins];
     unsigned long long sym_flop[kNumBins];
+    unsigned long long sym_annz[kNumBins];
     unsigned long long num_flop[kNumBins];
     unsigned long long num_nnz[kNumBins];
     unsigned long long num_annz[kNumBins];

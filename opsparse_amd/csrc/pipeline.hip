@@ -204,6 +204,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
         for (int b = 0; b < kNumBins; b++) {
             t->sym_rows[b] = st.sym_size[b];
             t->sym_flop[b] = static_cast<int64_t>(st.sym_flop[b]);
+            t->sym_arows_nnz[b] = static_cast<int64_t>(st.sym_annz[b]);
         }
     }
 
