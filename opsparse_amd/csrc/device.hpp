@@ -120,19 +120,127 @@ struct FullRow {
     }
 };
 
-// Calls f(b_index, a_value) for every product of A row segment [a_beg, a_end).
-// Unit `unit` of `nunits` units takes A chunks unit, unit+nunits, ...  All
-// lanes of a unit must call this together with the same arguments.
-// `range(e, k, bs, len)` gives the B entries [bs, bs+len) an A-nonzero e
-// (column k) contributes.
-template <int U, bool WITH_A, typename V, typename Range, typename F>
-__device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit, int nunits,
-                                                 const int *__restrict__ acol, const V *__restrict__ aval,
-                                                 const Range &range, F &&f)
+template <int U>
+__device__ __forceinline__ int unit_max(int v)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = U >> 1; d >= 1; d >>= 1) v = max(v, bperm(v, lane ^ d));
+    return v;
+}
+
+// Product consumers have two steps so the walkers below can issue the
+// global reads of two products before either LDS update:
+//   Item load(int b_index) const;  // global reads of one B entry
+//   void apply(const Item &, V a);  // LDS / atomic update with A's value
+
+// Grouped walk of one loaded chunk: GS lanes per A-nonzero, each group
+// strides over its B row two entries at a time.  No per-product cross-lane
+// search; used when the chunk's B rows have similar lengths.
+template <int U, int GS, bool WITH_A, typename V, typename Op>
+__device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op &op)
+{
+    static_assert(GS <= U, "group wider than unit");
+    const int lane = lane_id();
+    const int gl = lane & (U - 1);
+    const int gbase = lane & ~(U - 1);
+    constexpr int PER = U / GS;
+    const int sub = gl & (GS - 1);
+    const int grp = gl / GS;
+    for (int q0 = 0; q0 < cnt; q0 += PER) {
+        const int q = q0 + grp;
+        const int src = gbase + (q < U ? q : 0);
+        // Every bpermute runs with the whole unit active: a source lane that
+        // is inactive in the instruction would read back as 0.
+        const int qb = bperm(bs, src);
+        const int lq = bperm(len, src);
+        const int ql = q < cnt ? lq : 0;
+        V qa = V(0);
+        if constexpr (WITH_A) qa = bperm(av, src);
+        for (int t = sub; t < ql; t += 2 * GS) {
+            const bool two = t + GS < ql;
+            const typename Op::Item i0 = op.load(qb + t);
+            typename Op::Item i1{};
+            if (two) i1 = op.load(qb + t + GS);
+            op.apply(i0, qa);
+            if (two) op.apply(i1, qa);
+        }
+    }
+}
+
+// Largest q with excl[q] <= p over the unit (binary search, all lanes).
+template <int U>
+__device__ __forceinline__ void unit_locate(int excl, int gbase, int p0, int p1, int &q0, int &v0, int &q1, int &v1)
+{
+    q0 = 0;
+    v0 = 0;
+    q1 = 0;
+    v1 = 0;
+#pragma unroll
+    for (int s = U >> 1; s >= 1; s >>= 1) {
+        const int c0 = q0 + s, c1 = q1 + s;
+        const int e0 = bperm(excl, gbase + c0);
+        const int e1 = bperm(excl, gbase + c1);
+        if (e0 <= p0) {
+            q0 = c0;
+            v0 = e0;
+        }
+        if (e1 <= p1) {
+            q1 = c1;
+            v1 = e1;
+        }
+    }
+}
+
+// Flattened walk of one loaded chunk: lane l takes products l, l+U, ... of
+// the chunk (two per step) and finds each one's A-nonzero by binary search
+// over the scanned B-row lengths.  Balanced for any length distribution.
+template <int U, bool WITH_A, typename V, typename Op>
+__device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op)
 {
     const int lane = lane_id();
     const int gl = lane & (U - 1);
     const int gbase = lane & ~(U - 1);
+    const int incl = unit_incl_scan<U>(len);
+    const int excl = incl - len;
+    const int total = bperm(incl, gbase + U - 1);
+    for (int base = 0; base < total; base += 2 * U) {
+        const int p0 = base + gl, p1 = p0 + U;
+        int q0, v0, q1, v1;
+        unit_locate<U>(excl, gbase, p0, p1, q0, v0, q1, v1);
+        const int b0 = bperm(bs, gbase + q0);
+        const int b1 = bperm(bs, gbase + q1);
+        V a0 = V(0), a1 = V(0);
+        if constexpr (WITH_A) {
+            a0 = bperm(av, gbase + q0);
+            a1 = bperm(av, gbase + q1);
+        }
+        const bool ok0 = p0 < total, ok1 = p1 < total;
+        typename Op::Item i0{}, i1{};
+        if (ok0) i0 = op.load(b0 + (p0 - v0));
+        if (ok1) i1 = op.load(b1 + (p1 - v1));
+        if (ok0) op.apply(i0, a0);
+        if (ok1) op.apply(i1, a1);
+    }
+}
+
+// Calls the consumer for every product of A row segment [a_beg, a_end).
+// Unit `unit` of `nunits` units takes A chunks unit, unit+nunits, ...  All
+// lanes of a unit must call this together with the same arguments.
+// `range(e, k, bs, len)` gives the B entries [bs, bs+len) an A-nonzero e
+// (column k) contributes.
+//
+// Per chunk of U A-nonzeros the walk is chosen at run time: if the longest
+// B row is at most twice the mean, lanes are grouped per A-nonzero (group
+// width = next power of two of the mean, 4..U); otherwise products are
+// flattened across the unit.
+template <int U, bool WITH_A, typename V, typename Range, typename Op>
+__device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit, int nunits,
+                                                 const int *__restrict__ acol, const V *__restrict__ aval,
+                                                 const Range &range, Op &op)
+{
+    const int lane = lane_id();
+    const int gl = lane & (U - 1);
     for (int c = a_beg + unit * U; c < a_end; c += nunits * U) {
         const int e = c + gl;
         int bs = 0, len = 0;
@@ -142,25 +250,26 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
             range(e, k, bs, len);
             if constexpr (WITH_A) av = aval[e];
         }
-        const int incl = unit_incl_scan<U>(len);
-        const int excl = incl - len;
-        const int total = bperm(incl, gbase + U - 1);
-        for (int base = 0; base < total; base += U) {
-            const int p = base + gl;
-            int q = 0, qv = 0;
-#pragma unroll
-            for (int s = U >> 1; s >= 1; s >>= 1) {
-                const int cand = q + s;
-                const int v = bperm(excl, gbase + cand);
-                if (v <= p) {
-                    q = cand;
-                    qv = v;
-                }
+        const int cnt = min(U, a_end - c);
+        const int total = unit_sum<U>(len);
+        const int mx = unit_max<U>(len);
+        if (total == 0) continue;
+        const int mean = (total + cnt - 1) / cnt;
+        if (mx <= 2 * mean) {
+            const int gs = mean <= 4 ? 4 : (mean <= 8 ? 8 : (mean <= 16 ? 16 : (mean <= 32 ? 32 : 64)));
+            if (gs <= 4) {
+                chunk_grouped<U, 4, WITH_A, V>(cnt, bs, len, av, op);
+            } else if (gs == 8 || U == 8) {
+                chunk_grouped<U, 8, WITH_A, V>(cnt, bs, len, av, op);
+            } else if (gs == 16 || U == 16) {
+                if constexpr (U >= 16) chunk_grouped<U, 16, WITH_A, V>(cnt, bs, len, av, op);
+            } else if (gs == 32 || U == 32) {
+                if constexpr (U >= 32) chunk_grouped<U, 32, WITH_A, V>(cnt, bs, len, av, op);
+            } else {
+                if constexpr (U >= 64) chunk_grouped<U, 64, WITH_A, V>(cnt, bs, len, av, op);
             }
-            const int qb = bperm(bs, gbase + q);
-            V qa = V(0);
-            if constexpr (WITH_A) qa = bperm(av, gbase + q);
-            if (p < total) f(qb + (p - qv), qa);
+        } else {
+            chunk_flat<U, WITH_A, V>(bs, len, av, op);
         }
     }
 }
@@ -263,6 +372,44 @@ __device__ __forceinline__ void unit_bitonic_sort(int (&key)[K], V (&val)[K])
                         key[k] = ok;
                         val[k] = ov;
                     }
+                }
+            }
+        }
+    }
+}
+
+// Bitonic sort of K*U int keys held K per lane across a unit of U lanes;
+// element j = k*U + (lane % U) ends up holding the j-th smallest key.
+template <int U, int K>
+__device__ __forceinline__ void unit_bitonic_sort_keys(int (&key)[K])
+{
+    const int lane = lane_id();
+    const int gl = lane & (U - 1);
+    constexpr int NEL = K * U;
+#pragma unroll
+    for (int size = 2; size <= NEL; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= U) {
+                const int ks = stride / U;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int pk = k ^ ks;
+                    if (pk > k) {
+                        const bool asc = ((k * U + gl) & size) == 0;
+                        const int a = key[k], b = key[pk];
+                        key[k] = asc ? min(a, b) : max(a, b);
+                        key[pk] = asc ? max(a, b) : min(a, b);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int j = k * U + gl;
+                    const bool asc = (j & size) == 0;
+                    const bool lower = (j & stride) == 0;
+                    const int o = bperm(key[k], lane ^ stride);
+                    key[k] = (asc == lower) ? min(key[k], o) : max(key[k], o);
                 }
             }
         }

@@ -22,10 +22,45 @@ namespace opsamd {
 
 namespace {
 
-template <typename V>
-struct Prod {
+// Consumers of the product walk (device.hpp): accumulate a*b into the
+// row's hash table, or into a dense column window.
+template <int T, typename V>
+struct NumInsert {
     const int *__restrict__ bcol;
     const V *__restrict__ bval;
+    int *kt;
+    V *vt;
+    struct Item {
+        int c;
+        V b;
+    };
+    __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
+    __device__ __forceinline__ void apply(const Item &it, V a)
+    {
+        bool fresh;
+        const int h = hash_insert<T>(kt, it.c, fresh);
+        atomicAdd(vt + h, a * it.b);
+    }
+};
+
+template <typename V>
+struct DenseInsert {
+    const int *__restrict__ bcol;
+    const V *__restrict__ bval;
+    V *acc;
+    unsigned *bm;
+    int w0;
+    struct Item {
+        int c;
+        V b;
+    };
+    __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
+    __device__ __forceinline__ void apply(const Item &it, V a)
+    {
+        const int c = it.c - w0;
+        atomicAdd(acc + c, a * it.b);
+        atomicOr(bm + (c >> 5), 1u << (c & 31));
+    }
 };
 
 // Bins 0-4: G <= 64 lanes per row, BS/G rows per block, register sort.
@@ -44,6 +79,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     static_assert(G <= kWave && BS % G == 0 && K >= 1, "bad config");
     __shared__ int keys[RPB * T];
     __shared__ V vals[RPB * T];
+    __shared__ int ckeys[RPB * (T / 2)];
 
     const int tid = threadIdx.x;
     const int slot = tid / G;
@@ -61,67 +97,52 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     if (ridx < nrows) {
         r = rows ? rows[ridx] : ridx;
         const FullRow range{brpt};
-        for_each_product<U, true, V>(arpt[r], arpt[r + 1], 0, 1, acol, aval, range, [&](int k, V a) {
-            bool fresh;
-            const int h = hash_insert<T>(kt, bcol[k], fresh);
-            atomicAdd(vt + h, a * bval[k]);
-        });
+        NumInsert<T, V> op{bcol, bval, kt, vt};
+        for_each_product<U, true, V>(arpt[r], arpt[r + 1], 0, 1, acol, aval, range, op);
     }
     __syncthreads();
     if (ridx >= nrows) return;
 
-    // Compact the row's slots in place, in slot order.
-    int kk[S];
-    V vv[S];
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-        kk[s] = kt[s * U + gl];
-        vv[s] = vt[s * U + gl];
-    }
-    wave_lds_sync();
+    // Compact the row's keys (slot order) into its key list; the table stays
+    // intact and supplies the values after the keys are sorted.
+    int *ck = ckeys + slot * (T / 2);
     int n = 0;
     const unsigned long long below = gl ? ((1ull << gl) - 1ull) : 0ull;
 #pragma unroll
     for (int s = 0; s < S; s++) {
-        const bool valid = kk[s] != -1;
+        const int key = kt[s * U + gl];
+        const bool valid = key != -1;
         const unsigned long long m = unit_ballot<U>(valid);
-        if (valid) {
-            const int pos = n + __popcll(m & below);
-            kt[pos] = kk[s];
-            vt[pos] = vv[s];
-        }
+        const int pos = n + __popcll(m & below);
+        if (valid && pos < T / 2) ck[pos] = key;
         n += __popcll(m);
     }
+    n = min(n, T / 2);
     wave_lds_sync();
 
     int sk[K];
-    V sv[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = k * U + gl;
-        if (j < n) {
-            sk[k] = kt[j];
-            sv[k] = vt[j];
-        } else {
-            sk[k] = INT_MAX;
-            sv[k] = V(0);
-        }
+        sk[k] = j < n ? ck[j] : INT_MAX;
     }
-    unit_bitonic_sort<U, K, V>(sk, sv);
+    unit_bitonic_sort_keys<U, K>(sk);
     const long long off = crpt[r];
     const int n_out = min(n, static_cast<int>(crpt[r + 1] - off));  // never write past the row
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = k * U + gl;
         if (j < n_out) {
+            const int h = hash_find<T>(kt, sk[k]);
             ccol[off + j] = sk[k];
-            cval[off + j] = sv[k];
+            cval[off + j] = h >= 0 ? vt[h] : V(0);
         }
     }
 }
 
 // Bins 5-6: one row per block of BS threads; table of T slots in dynamic
-// LDS, then compaction to (key << 32 | slot) words and an LDS bitonic sort.
+// LDS, then compaction of the keys and an LDS bitonic sort of the keys; the
+// values are read back from the table by key.
 template <int T, int BS, typename V>
 __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                   const V *__restrict__ aval, const int *__restrict__ brpt,
@@ -131,9 +152,9 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
                                                   V *__restrict__ cval)
 {
     extern __shared__ unsigned long long smem_u64[];
-    unsigned long long *sk = smem_u64;                               // T/2 sort words
-    V *vt = reinterpret_cast<V *>(smem_u64 + T / 2);                 // T values
-    int *kt = reinterpret_cast<int *>(vt + T);                       // T keys
+    V *vt = reinterpret_cast<V *>(smem_u64);   // T values
+    int *kt = reinterpret_cast<int *>(vt + T);  // T keys
+    int *sk = kt + T;                           // T/2 sorted keys
     __shared__ int s_pos;
     constexpr int NU = BS / kWave;
     const int tid = threadIdx.x;
@@ -148,11 +169,8 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
         if (tid == 0) s_pos = 0;
         __syncthreads();
         const FullRow range{brpt};
-        for_each_product<kWave, true, V>(arpt[r], arpt[r + 1], wave, NU, acol, aval, range, [&](int k, V a) {
-            bool fresh;
-            const int h = hash_insert<T>(kt, bcol[k], fresh);
-            atomicAdd(vt + h, a * bval[k]);
-        });
+        NumInsert<T, V> op{bcol, bval, kt, vt};
+        for_each_product<kWave, true, V>(arpt[r], arpt[r + 1], wave, NU, acol, aval, range, op);
         __syncthreads();
         const long long off = crpt[r];
         const int n = min(static_cast<int>(crpt[r + 1] - off), T / 2);  // rows of this bin have n <= T/2
@@ -164,14 +182,12 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
             if (lane_id() == 0 && m) base = atomicAdd(&s_pos, __popcll(m));
             base = bperm(base, 0);
             const int pos = base + __popcll(m & ((1ull << lane_id()) - 1ull));
-            if (valid && pos < T / 2) {
-                sk[pos] = (static_cast<unsigned long long>(static_cast<unsigned>(key)) << 32) | static_cast<unsigned>(i);
-            }
+            if (valid && pos < T / 2) sk[pos] = key;
         }
         int p2 = 1;
         while (p2 < n) p2 <<= 1;
         __syncthreads();
-        for (int i = n + tid; i < p2; i += BS) sk[i] = ~0ull;
+        for (int i = n + tid; i < p2; i += BS) sk[i] = INT_MAX;
         __syncthreads();
         for (int size = 2; size <= p2; size <<= 1) {
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -179,7 +195,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
                     const int lo = 2 * stride * (i / stride) + (i % stride);
                     const int hi = lo + stride;
                     const bool asc = (lo & size) == 0;
-                    const unsigned long long a = sk[lo], b = sk[hi];
+                    const int a = sk[lo], b = sk[hi];
                     if ((a > b) == asc) {
                         sk[lo] = b;
                         sk[hi] = a;
@@ -189,9 +205,10 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
             }
         }
         for (int j = tid; j < n; j += BS) {
-            const unsigned long long w = sk[j];
-            ccol[off + j] = static_cast<int>(w >> 32);
-            cval[off + j] = vt[static_cast<unsigned>(w) & (T - 1)];
+            const int key = sk[j];
+            const int h = hash_find<T>(kt, key);
+            ccol[off + j] = key;
+            cval[off + j] = h >= 0 ? vt[h] : V(0);
         }
         __syncthreads();
     }
@@ -298,11 +315,8 @@ __global__ __launch_bounds__(kLongBS) void k_num_long(const int *__restrict__ ar
             const int wend = min(ncols, w0 + D::W);
             int nxt = INT_MAX;
             const CursorRange range{brpt, bcol, cursor, wend, first, wend >= ncols, &nxt};
-            for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, [&](int k, V a) {
-                const int c = bcol[k] - w0;
-                atomicAdd(acc + c, a * bval[k]);
-                atomicOr(bm + (c >> 5), 1u << (c & 31));
-            });
+            DenseInsert<V> op{bcol, bval, acc, bm, w0};
+            for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op);
             nxt = unit_min<kWave>(nxt);
             if (lane_id() == 0) s_min[wave] = nxt;
             __syncthreads();
@@ -351,7 +365,7 @@ void launch_small(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, co
 template <int T, int BS, typename V>
 constexpr size_t block_lds()
 {
-    return sizeof(unsigned long long) * (T / 2) + sizeof(V) * T + sizeof(int) * T;
+    return sizeof(V) * T + sizeof(int) * T + sizeof(int) * (T / 2);
 }
 
 template <int T, int BS, typename V>

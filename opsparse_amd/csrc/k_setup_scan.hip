@@ -5,6 +5,8 @@
 // k_symbolic_binning / k_symbolic_binning2 (kernels/binning.cuh:11-81),
 // k_numeric_binning / k_numeric_binning2 (binning.cuh:90-174) and the CUB
 // exclusive scan of C.d_rpt (spgemm.cu:49, 113).
+#include <algorithm>
+
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -34,71 +36,107 @@ __global__ void k_zero_stats(DevStats *st)
     for (unsigned i = threadIdx.x; i < sizeof(DevStats) / 4; i += blockDim.x) p[i] = 0;
 }
 
-// One block per tile of 256 rows.  A-nonzeros of the tile are spread over
-// the threads (so a 40k-entry row costs the same as 40k short rows), each
-// adds its B-row length to its row's LDS counter.
+// Products per row, grid-stride over tiles of 256 rows.  Each wave takes
+// 64 rows of the tile in rounds of 8 rows, 8 lanes per row striding over the
+// row's A-nonzeros (coalesced reads of A.col), then an 8-lane sum.  Rows with
+// more than 256 A-nonzeros are summed by the whole block afterwards, so one
+// 40k-entry row neither serialises a group nor stalls its wave.  Per-bin
+// counts accumulate in LDS and reach global memory once per block.
 constexpr int kSetupRows = 256;
+constexpr int kSetupMaxGrid = 2048;
+constexpr int kSetupGroup = 8;
+constexpr int kLongRowA = 256;
 
 __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const int *__restrict__ brpt, int M, int *__restrict__ row_flop,
                                                       int *__restrict__ row_nnz, DevStats *__restrict__ st)
 {
-    __shared__ int s_rpt[kSetupRows + 1];
-    __shared__ unsigned long long s_flop[kSetupRows];
     __shared__ int s_hist[kNumBins];
     __shared__ unsigned long long s_bflop[kNumBins];
     __shared__ unsigned long long s_bannz[kNumBins];
     __shared__ unsigned long long s_red[kSetupRows / kWave];
+    __shared__ unsigned long long s_f[kSetupRows];
+    __shared__ int s_long[kSetupRows];
+    __shared__ int s_nlong;
     __shared__ int s_max;
 
     const int tid = threadIdx.x;
-    const int r0 = blockIdx.x * kSetupRows;
-    const int nr = min(kSetupRows, M - r0);
-    if (tid < nr) s_rpt[tid] = arpt[r0 + tid];
-    if (tid == 0) s_rpt[nr] = arpt[r0 + nr];
-    s_flop[tid] = 0;
+    const int lane = lane_id();
+    const int wave = tid / kWave;
+    const int sub = lane & (kSetupGroup - 1);
     if (tid < kNumBins) {
         s_hist[tid] = 0;
         s_bflop[tid] = 0;
         s_bannz[tid] = 0;
     }
-    if (tid == 0) s_max = 0;
-    __syncthreads();
-
-    const int e_beg = s_rpt[0];
-    const int e_end = s_rpt[nr];
-    for (int e = e_beg + tid; e < e_end; e += kSetupRows) {
-        const int k = acol[e];
-        const int len = brpt[k + 1] - brpt[k];
-        // largest i in [0, nr) with s_rpt[i] <= e
-        int lo = 0, hi = nr - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_rpt[mid] <= e)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        atomicAdd(&s_flop[lo], static_cast<unsigned long long>(len));
+    if (tid == 0) {
+        s_max = 0;
+        s_nlong = 0;
     }
     __syncthreads();
 
-    unsigned long long f = 0;
-    if (tid < nr) {
-        f = s_flop[tid];
-        const int fi = f > static_cast<unsigned long long>(INT_MAX) ? INT_MAX : static_cast<int>(f);
-        row_flop[r0 + tid] = fi;
-        if (fi == 0) {
-            row_nnz[r0 + tid] = 0;
-        } else {
-            const int b = dev_sym_bin(fi);
-            atomicAdd(&s_hist[b], 1);
-            atomicAdd(&s_bflop[b], f);
-            atomicAdd(&s_bannz[b], static_cast<unsigned long long>(s_rpt[tid + 1] - s_rpt[tid]));
+    unsigned long long tot = 0;
+    int mx = 0;
+    const int ntiles = (M + kSetupRows - 1) / kSetupRows;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int r0 = tile * kSetupRows;
+        for (int round = 0; round < kWave / kSetupGroup; round++) {
+            const int rl = wave * kWave + round * kSetupGroup + lane / kSetupGroup;
+            const int r = r0 + rl;
+            int a0 = 0, a1 = 0;
+            if (r < M) {
+                a0 = arpt[r];
+                a1 = arpt[r + 1];
+            }
+            const bool longrow = a1 - a0 > kLongRowA;
+            unsigned long long part = 0;
+            if (!longrow) {
+                for (int e = a0 + sub; e < a1; e += kSetupGroup) {
+                    const int k = acol[e];
+                    part += static_cast<unsigned long long>(brpt[k + 1] - brpt[k]);
+                }
+            }
+            part = unit_sum<kSetupGroup>(part);
+            if (sub == 0) {
+                s_f[rl] = part;
+                if (longrow) s_long[atomicAdd(&s_nlong, 1)] = rl;
+            }
         }
-        atomicMax(&s_max, fi);
+        __syncthreads();
+        const int nlong = s_nlong;
+        for (int i = 0; i < nlong; i++) {
+            const int rl = s_long[i];
+            const int b0 = arpt[r0 + rl], b1 = arpt[r0 + rl + 1];
+            unsigned long long part = 0;
+            for (int e = b0 + tid; e < b1; e += kSetupRows) {
+                const int k = acol[e];
+                part += static_cast<unsigned long long>(brpt[k + 1] - brpt[k]);
+            }
+            part = block_sum<kSetupRows>(part, s_red);
+            if (tid == 0) s_f[rl] = part;
+        }
+        __syncthreads();
+        const int r = r0 + tid;
+        if (r < M) {
+            const unsigned long long f = s_f[tid];
+            const int fi = f > static_cast<unsigned long long>(INT_MAX) ? INT_MAX : static_cast<int>(f);
+            row_flop[r] = fi;
+            if (fi == 0) {
+                row_nnz[r] = 0;
+            } else {
+                const int b = dev_sym_bin(fi);
+                atomicAdd(&s_hist[b], 1);
+                atomicAdd(&s_bflop[b], f);
+                atomicAdd(&s_bannz[b], static_cast<unsigned long long>(arpt[r + 1] - arpt[r]));
+            }
+            mx = max(mx, fi);
+            tot += f;
+        }
+        if (tid == 0) s_nlong = 0;
+        __syncthreads();
     }
-    const unsigned long long tot = block_sum<kSetupRows>(f, s_red);
+    atomicMax(&s_max, mx);
+    tot = block_sum<kSetupRows>(tot, s_red);
     if (tid < kNumBins && s_hist[tid]) {
         atomicAdd(&st->sym_size[tid], s_hist[tid]);
         atomicAdd(&st->sym_flop[tid], s_bflop[tid]);
@@ -111,23 +149,43 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
 }
 
 // Scatter row ids into bin lists.  Bin b occupies [sum(size[0..b)), ...).
-// Order inside a bin depends on block scheduling, as in the reference
-// (binning.cuh:75); only the set of rows per bin is fixed.
-__global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ key, int M, int which,
-                                                     DevStats *__restrict__ st, int *__restrict__ bins)
+// Tiles of 4096 rows; ranks inside a tile come from wave ballots, and each
+// tile reserves its slice of every bin with one global atomic.  Order inside
+// a bin depends on tile scheduling, as in the reference (binning.cuh:75);
+// only the set of rows per bin is fixed.
+constexpr int kScatBS = 1024;
+constexpr int kScatItems = 4;
+
+__global__ __launch_bounds__(kScatBS) void k_bin_scatter(const int *__restrict__ key, int M, int which,
+                                                         DevStats *__restrict__ st, int *__restrict__ bins)
 {
     __shared__ int s_cnt[kNumBins];
     __shared__ int s_base[kNumBins];
     const int tid = threadIdx.x;
-    const int r = blockIdx.x * 256 + tid;
+    const int lane = lane_id();
+    const unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
     if (tid < kNumBins) s_cnt[tid] = 0;
     __syncthreads();
-    int b = -1, local = 0;
-    if (r < M) {
-        const int v = key[r];
-        if (v > 0) {
-            b = which ? dev_num_bin(v) : dev_sym_bin(v);
-            local = atomicAdd(&s_cnt[b], 1);
+    const int r0 = blockIdx.x * (kScatBS * kScatItems);
+    int bin[kScatItems], rank[kScatItems];
+#pragma unroll
+    for (int i = 0; i < kScatItems; i++) {
+        const int r = r0 + i * kScatBS + tid;
+        bin[i] = -1;
+        rank[i] = 0;
+        if (r < M) {
+            const int v = key[r];
+            if (v > 0) bin[i] = which ? dev_num_bin(v) : dev_sym_bin(v);
+        }
+#pragma unroll
+        for (int b = 0; b < kNumBins; b++) {
+            const unsigned long long m = __ballot(bin[i] == b);
+            if (!m) continue;
+            const int leader = __ffsll(static_cast<long long>(m)) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&s_cnt[b], __popcll(m));
+            base = __builtin_amdgcn_readlane(base, leader);
+            if (bin[i] == b) rank[i] = base + __popcll(m & lt);
         }
     }
     __syncthreads();
@@ -139,7 +197,9 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ key
         s_base[tid] = s_cnt[tid] ? off + atomicAdd(&fill[tid], s_cnt[tid]) : 0;
     }
     __syncthreads();
-    if (b >= 0) bins[s_base[b] + local] = r;
+#pragma unroll
+    for (int i = 0; i < kScatItems; i++)
+        if (bin[i] >= 0) bins[s_base[bin[i]] + rank[i]] = r0 + i * kScatBS + tid;
 }
 
 // --- scan: 1024 threads x 4 rows per block ---
@@ -317,14 +377,15 @@ void launch_zero_stats(DevStats *st, hipStream_t s) { k_zero_stats<<<1, 256, 0, 
 void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, hipStream_t s)
 {
     if (A.M == 0) return;
-    const int grid = (A.M + kSetupRows - 1) / kSetupRows;
+    const int grid = std::min((A.M + kSetupRows - 1) / kSetupRows, kSetupMaxGrid);
     k_setup<<<grid, kSetupRows, 0, s>>>(A.rpt, A.col, B.rpt, A.M, row_flop, row_nnz, st);
 }
 
 void launch_bin_scatter(const int *key, int M, int which, DevStats *st, int *bins, hipStream_t s)
 {
     if (M == 0) return;
-    k_bin_scatter<<<(M + 255) / 256, 256, 0, s>>>(key, M, which, st, bins);
+    constexpr int tile = kScatBS * kScatItems;
+    k_bin_scatter<<<(M + tile - 1) / tile, kScatBS, 0, s>>>(key, M, which, st, bins);
 }
 
 int scan_blocks(int M) { return (M + kScanTile - 1) / kScanTile; }

@@ -12,12 +12,47 @@
 // atomic per insert.
 // Bin 7 (more than 16384 products): an LDS bitmap over up to 2^20 columns
 // per window; a set bit is a column of C, counted when its atomicOr flips it.
+#include <algorithm>
+
 #include "device.hpp"
 #include "launch.hpp"
 
 namespace opsamd {
 
 namespace {
+
+// Consumers of the product walk (device.hpp): insert B's column into the
+// row's hash table / bitmap and count the keys that were new.
+template <int T>
+struct SymInsert {
+    const int *__restrict__ bcol;
+    int *tab;
+    int cnt = 0;
+    using Item = int;
+    __device__ __forceinline__ int load(int k) const { return bcol[k]; }
+    __device__ __forceinline__ void apply(int c, double)
+    {
+        bool fresh;
+        hash_insert<T>(tab, c, fresh);
+        cnt += fresh ? 1 : 0;
+    }
+};
+
+struct BitmapInsert {
+    const int *__restrict__ bcol;
+    unsigned *bm;
+    int w0;
+    int cnt = 0;
+    using Item = int;
+    __device__ __forceinline__ int load(int k) const { return bcol[k]; }
+    __device__ __forceinline__ void apply(int col, double)
+    {
+        const int c = col - w0;
+        const unsigned bit = 1u << (c & 31);
+        const unsigned old = atomicOr(&bm[c >> 5], bit);
+        cnt += (old & bit) ? 0 : 1;
+    }
+};
 
 // G lanes per row (G <= 64: BS/G rows per block; G == BS > 64: one row per
 // block, G/64 waves share the table).
@@ -45,14 +80,11 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     int r = 0;
     if (ridx < nrows) {
         r = rows ? rows[ridx] : ridx;
-        int *t = tab + slot * T;
+        SymInsert<T> op{bcol, tab + slot * T};
         const FullRow range{brpt};
         for_each_product<U, false, double>(arpt[r], arpt[r + 1], unit, NU, acol, static_cast<const double *>(nullptr),
-                                           range, [&](int k, double) {
-                                               bool fresh;
-                                               hash_insert<T>(t, bcol[k], fresh);
-                                               cnt += fresh ? 1 : 0;
-                                           });
+                                           range, op);
+        cnt = op.cnt;
     }
     if constexpr (NU == 1) {
         cnt = unit_sum<U>(cnt);
@@ -109,7 +141,7 @@ struct WindowRange {
 __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const int *__restrict__ brpt, const int *__restrict__ bcol,
                                                       const int *__restrict__ rows, int nrows, int ncols,
-                                                      int *__restrict__ row_nnz)
+                                                      int wcols, int *__restrict__ row_nnz)
 {
     extern __shared__ unsigned bm[];
     __shared__ int s_red[kLongBS / kWave];
@@ -120,19 +152,16 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
         const int r = rows ? rows[ridx] : ridx;
         const int a_beg = arpt[r], a_end = arpt[r + 1];
         int cnt = 0;
-        for (int w0 = 0; w0 < ncols; w0 += kBitmapCols) {
-            const int wend = min(ncols, w0 + kBitmapCols);
+        for (int w0 = 0; w0 < ncols; w0 += wcols) {
+            const int wend = min(ncols, w0 + wcols);
             const int nwords = (wend - w0 + 31) >> 5;
             for (int i = tid; i < nwords; i += kLongBS) bm[i] = 0u;
             __syncthreads();
             const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols};
+            BitmapInsert op{bcol, bm, w0};
             for_each_product<kWave, false, double>(a_beg, a_end, wave, NW, acol, static_cast<const double *>(nullptr),
-                                                   range, [&](int k, double) {
-                                                       const int c = bcol[k] - w0;
-                                                       const unsigned bit = 1u << (c & 31);
-                                                       const unsigned old = atomicOr(&bm[c >> 5], bit);
-                                                       cnt += (old & bit) ? 0 : 1;
-                                                   });
+                                                   range, op);
+            cnt += op.cnt;
             __syncthreads();
         }
         const int tot = block_sum<kLongBS>(cnt, s_red);
@@ -170,10 +199,15 @@ void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *row
         case 4: launch_hash<64, 1024, 256>(A, B, rows, nrows, row_nnz, s); break;
         case 5: launch_hash<256, 4096, 256>(A, B, rows, nrows, row_nnz, s); break;
         case 6: launch_hash<1024, 16384, 1024>(A, B, rows, nrows, row_nnz, s); break;
-        default:
-            k_sym_long<<<nrows, kLongBS, symbolic_long_lds(), s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, B.N,
-                                                                  row_nnz);
+        default: {
+            // Bitmap sized to the column count (up to 2^20 columns per window),
+            // so narrow products keep more blocks resident per CU.
+            const int words = static_cast<int>(std::min<long long>(kBitmapWords, (static_cast<long long>(B.N) + 31) / 32));
+            const int wcols = words * 32;
+            k_sym_long<<<nrows, kLongBS, sizeof(unsigned) * static_cast<size_t>(words), s>>>(
+                A.rpt, A.col, B.rpt, B.col, rows, nrows, B.N, wcols, row_nnz);
             break;
+        }
     }
 }
 
