@@ -256,7 +256,10 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
         if (total == 0) continue;
         const int mean = (total + cnt - 1) / cnt;
         if (mx <= 2 * mean) {
-            const int gs = mean <= 4 ? 4 : (mean <= 8 ? 8 : (mean <= 16 ? 16 : (mean <= 32 ? 32 : 64)));
+            // Groups of about half the mean length: several products per lane per
+            // round amortise the per-round cross-lane reads.
+            const int half = mean >> 1;
+            const int gs = half >= 64 ? 64 : (half >= 32 ? 32 : (half >= 16 ? 16 : (half >= 8 ? 8 : 4)));
             if (gs <= 4) {
                 chunk_grouped<U, 4, WITH_A, V>(cnt, bs, len, av, op);
             } else if (gs == 8 || U == 8) {
@@ -274,20 +277,28 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
     }
 }
 
+// Home slot of a column key: the reference's multiplicative hash
+// (col * 107, symbolic.cuh:98) on the low 24 bits of the key, so it maps to
+// the full-rate v_mul_u32_u24 instead of the quarter-rate 32-bit multiply.
+__device__ __forceinline__ unsigned hash_slot(int key) { return __umul24(static_cast<unsigned>(key), kHashMul); }
+
 // Linear-probing insert of `key` (>= 0) into an int table of T (power of 2)
 // slots initialised to -1.  Returns the slot; `fresh` tells whether this
-// call created the key.
+// call created the key.  A plain LDS read comes first: most products of a
+// row repeat a key that is already in the table, and ds_read is cheaper than
+// ds_cmpst; the CAS is only issued on an empty slot.
 template <int T>
 __device__ __forceinline__ int hash_insert(int *tab, int key, bool &fresh)
 {
-    unsigned h = (static_cast<unsigned>(key) * kHashMul) & (T - 1);
+    unsigned h = hash_slot(key) & (T - 1);
     while (true) {
-        const int old = atomicCAS(tab + h, -1, key);
-        if (old == -1) {
+        int cur = tab[h];
+        if (cur == -1) cur = atomicCAS(tab + h, -1, key);
+        if (cur == -1) {
             fresh = true;
             return static_cast<int>(h);
         }
-        if (old == key) {
+        if (cur == key) {
             fresh = false;
             return static_cast<int>(h);
         }
@@ -295,10 +306,22 @@ __device__ __forceinline__ int hash_insert(int *tab, int key, bool &fresh)
     }
 }
 
+// Blocks are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8).  With gridDim.x a multiple of 8, this returns a logical tile index
+// such that each XCD walks one contiguous range of tiles, so rows that share
+// B rows meet in the same L2.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_tile()
+{
+    const int per = gridDim.x / kNumXcd;
+    return (static_cast<int>(blockIdx.x) % kNumXcd) * per + static_cast<int>(blockIdx.x) / kNumXcd;
+}
+__host__ __device__ constexpr int xcd_grid(int tiles) { return (tiles + kNumXcd - 1) / kNumXcd * kNumXcd; }
+
 // Same with a runtime power-of-two mask.
 __device__ __forceinline__ int hash_insert_mask(int *tab, int key, unsigned mask, bool &fresh)
 {
-    unsigned h = (static_cast<unsigned>(key) * kHashMul) & mask;
+    unsigned h = hash_slot(key) & mask;
     while (true) {
         const int old = atomicCAS(tab + h, -1, key);
         if (old == -1) {
@@ -318,7 +341,7 @@ __device__ __forceinline__ int hash_insert_mask(int *tab, int key, unsigned mask
 template <int T>
 __device__ __forceinline__ int hash_find(const int *tab, int key)
 {
-    unsigned h = (static_cast<unsigned>(key) * kHashMul) & (T - 1);
+    unsigned h = hash_slot(key) & (T - 1);
     for (int i = 0; i < T; i++) {
         const int v = tab[h];
         if (v == key) return static_cast<int>(h);

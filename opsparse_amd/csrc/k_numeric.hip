@@ -90,7 +90,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     }
     __syncthreads();
 
-    const int ridx = blockIdx.x * RPB + slot;
+    const int ridx = xcd_tile() * RPB + slot;
     int r = 0;
     int *kt = keys + slot * T;
     V *vt = vals + slot * T;
@@ -357,7 +357,7 @@ void launch_small(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, co
                   V *cval, hipStream_t s)
 {
     constexpr int RPB = BS / G;
-    const int grid = (nrows + RPB - 1) / RPB;
+    const int grid = xcd_grid((nrows + RPB - 1) / RPB);
     k_num_small<G, T, BS, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col,
                                                 static_cast<const V *>(B.val), rows, nrows, crpt, ccol, cval);
 }

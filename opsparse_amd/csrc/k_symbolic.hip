@@ -75,7 +75,7 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     if (NU > 1 && tid < RPB) s_cnt[tid] = 0;
     __syncthreads();
 
-    const int ridx = blockIdx.x * RPB + slot;
+    const int ridx = xcd_tile() * RPB + slot;
     int cnt = 0;
     int r = 0;
     if (ridx < nrows) {
@@ -173,7 +173,7 @@ template <int G, int T, int BS>
 void launch_hash(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, hipStream_t s)
 {
     constexpr int RPB = BS / G;
-    const int grid = (nrows + RPB - 1) / RPB;
+    const int grid = xcd_grid((nrows + RPB - 1) / RPB);
     k_sym_hash<G, T, BS><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
 }
 
