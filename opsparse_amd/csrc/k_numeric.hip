@@ -77,17 +77,16 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     constexpr int S = T / U;        // table slots per lane
     constexpr int K = T / (2 * U);  // sorted entries per lane (n <= T/2)
     static_assert(G <= kWave && BS % G == 0 && K >= 1, "bad config");
-    __shared__ int keys[RPB * T];
-    __shared__ V vals[RPB * T];
+    __shared__ __align__(16) int keys[RPB * T];
+    __shared__ __align__(16) V vals[RPB * T];
     __shared__ int ckeys[RPB * (T / 2)];
 
     const int tid = threadIdx.x;
     const int slot = tid / G;
     const int gl = tid % G;
-    for (int i = tid; i < RPB * T; i += BS) {
-        keys[i] = -1;
-        vals[i] = V(0);
-    }
+    for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(keys)[i] = make_int4(-1, -1, -1, -1);
+    for (int i = tid; i < RPB * T * static_cast<int>(sizeof(V)) / 16; i += BS)
+        reinterpret_cast<int4 *>(vals)[i] = make_int4(0, 0, 0, 0);
     __syncthreads();
 
     const int ridx = xcd_tile() * RPB + slot;

@@ -65,13 +65,13 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     constexpr int RPB = BS / G;
     constexpr int NU = G / U;
     static_assert(BS % G == 0 && (T & (T - 1)) == 0, "bad config");
-    __shared__ int tab[RPB * T];
+    __shared__ __align__(16) int tab[RPB * T];
     __shared__ int s_cnt[RPB];
 
     const int tid = threadIdx.x;
     const int slot = tid / G;
     const int unit = (tid % G) / U;
-    for (int i = tid; i < RPB * T; i += BS) tab[i] = -1;
+    for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(tab)[i] = make_int4(-1, -1, -1, -1);
     if (NU > 1 && tid < RPB) s_cnt[tid] = 0;
     __syncthreads();
 
