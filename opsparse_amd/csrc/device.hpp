@@ -23,8 +23,6 @@ namespace opsamd {
 
 constexpr int kWave = 64;
 
-// Multiplicative hash constant of the reference (utils/config.hpp:32, kHashScale).
-constexpr unsigned kHashMul = 107u;
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(__lane_id()); }
 
@@ -138,7 +136,7 @@ __device__ __forceinline__ int unit_max(int v)
 // strides over its B row two entries at a time.  No per-product cross-lane
 // search; used when the chunk's B rows have similar lengths.
 template <int U, int GS, bool WITH_A, typename V, typename Op>
-__device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op &op)
+__device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op &op, int part, int nparts)
 {
     static_assert(GS <= U, "group wider than unit");
     const int lane = lane_id();
@@ -147,7 +145,7 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
     constexpr int PER = U / GS;
     const int sub = gl & (GS - 1);
     const int grp = gl / GS;
-    for (int q0 = 0; q0 < cnt; q0 += PER) {
+    for (int q0 = part * PER; q0 < cnt; q0 += nparts * PER) {
         const int q = q0 + grp;
         const int src = gbase + (q < U ? q : 0);
         // Every bpermute runs with the whole unit active: a source lane that
@@ -196,7 +194,7 @@ __device__ __forceinline__ void unit_locate(int excl, int gbase, int p0, int p1,
 // the chunk (two per step) and finds each one's A-nonzero by binary search
 // over the scanned B-row lengths.  Balanced for any length distribution.
 template <int U, bool WITH_A, typename V, typename Op>
-__device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op)
+__device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op, int part, int nparts)
 {
     const int lane = lane_id();
     const int gl = lane & (U - 1);
@@ -204,7 +202,7 @@ __device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op)
     const int incl = unit_incl_scan<U>(len);
     const int excl = incl - len;
     const int total = bperm(incl, gbase + U - 1);
-    for (int base = 0; base < total; base += 2 * U) {
+    for (int base = part * 2 * U; base < total; base += nparts * 2 * U) {
         const int p0 = base + gl, p1 = p0 + U;
         int q0, v0, q1, v1;
         unit_locate<U>(excl, gbase, p0, p1, q0, v0, q1, v1);
@@ -225,15 +223,20 @@ __device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op)
 }
 
 // Calls the consumer for every product of A row segment [a_beg, a_end).
-// Unit `unit` of `nunits` units takes A chunks unit, unit+nunits, ...  All
-// lanes of a unit must call this together with the same arguments.
+// `unit` of `nunits` units (all lanes of every unit must call this with the
+// same arguments) share the work in one of two ways:
+//  * rows with at least nunits*U A-nonzeros: unit u takes A chunks u,
+//    u+nunits, ...;
+//  * shorter rows (few A-nonzeros, possibly long B rows): every unit loads
+//    every chunk and takes an interleaved slice of its products, so all
+//    waves of a block stay busy.
 // `range(e, k, bs, len)` gives the B entries [bs, bs+len) an A-nonzero e
 // (column k) contributes.
 //
 // Per chunk of U A-nonzeros the walk is chosen at run time: if the longest
 // B row is at most twice the mean, lanes are grouped per A-nonzero (group
-// width = next power of two of the mean, 4..U); otherwise products are
-// flattened across the unit.
+// width about half the mean, 4..U); otherwise products are flattened
+// across the unit.
 template <int U, bool WITH_A, typename V, typename Range, typename Op>
 __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit, int nunits,
                                                  const int *__restrict__ acol, const V *__restrict__ aval,
@@ -241,7 +244,12 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
 {
     const int lane = lane_id();
     const int gl = lane & (U - 1);
-    for (int c = a_beg + unit * U; c < a_end; c += nunits * U) {
+    const bool shared = nunits > 1 && (a_end - a_beg) < nunits * U;
+    const int c0 = shared ? a_beg : a_beg + unit * U;
+    const int cstep = shared ? U : nunits * U;
+    const int part = shared ? unit : 0;
+    const int nparts = shared ? nunits : 1;
+    for (int c = c0; c < a_end; c += cstep) {
         const int e = c + gl;
         int bs = 0, len = 0;
         V av = V(0);
@@ -261,36 +269,59 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
             const int half = mean >> 1;
             const int gs = half >= 64 ? 64 : (half >= 32 ? 32 : (half >= 16 ? 16 : (half >= 8 ? 8 : 4)));
             if (gs <= 4) {
-                chunk_grouped<U, 4, WITH_A, V>(cnt, bs, len, av, op);
+                chunk_grouped<U, 4, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
             } else if (gs == 8 || U == 8) {
-                chunk_grouped<U, 8, WITH_A, V>(cnt, bs, len, av, op);
+                chunk_grouped<U, 8, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
             } else if (gs == 16 || U == 16) {
-                if constexpr (U >= 16) chunk_grouped<U, 16, WITH_A, V>(cnt, bs, len, av, op);
+                if constexpr (U >= 16) chunk_grouped<U, 16, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
             } else if (gs == 32 || U == 32) {
-                if constexpr (U >= 32) chunk_grouped<U, 32, WITH_A, V>(cnt, bs, len, av, op);
+                if constexpr (U >= 32) chunk_grouped<U, 32, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
             } else {
-                if constexpr (U >= 64) chunk_grouped<U, 64, WITH_A, V>(cnt, bs, len, av, op);
+                if constexpr (U >= 64) chunk_grouped<U, 64, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
             }
         } else {
-            chunk_flat<U, WITH_A, V>(bs, len, av, op);
+            chunk_flat<U, WITH_A, V>(bs, len, av, op, part, nparts);
         }
     }
 }
 
-// Home slot of a column key: the reference's multiplicative hash
-// (col * 107, symbolic.cuh:98) on the low 24 bits of the key, so it maps to
-// the full-rate v_mul_u32_u24 instead of the quarter-rate 32-bit multiply.
-__device__ __forceinline__ unsigned hash_slot(int key) { return __umul24(static_cast<unsigned>(key), kHashMul); }
+// Home slot of a column key, two families:
+//  * LIN: the reference's multiplicative hash col * 107 mod T
+//    (symbolic.cuh:98) on the full-rate 24-bit multiply.  Consecutive
+//    columns land 107 slots (11 LDS banks) apart, which makes it nearly
+//    collision- and conflict-free on rows made of runs of adjacent columns
+//    (stencils, banded and uniformly random patterns): used for the short-row
+//    bins.
+//  * FIB: multiplicative (Fibonacci) hashing in 24-bit words,
+//    h = ((col * 0x9E3779) mod 2^24) >> (24 - log2 T), 0x9E3779 = 2^24 /
+//    golden ratio.  Its top bits depend on every one of the key's low 24
+//    bits; power-law (R-MAT) column indices have strongly skewed low bits,
+//    which piles keys onto a few home slots under LIN.  Used for the long-row
+//    bins, where power-law rows put most of their products.
+enum HashKind { kHashLin = 0, kHashFib = 1 };
+constexpr unsigned kLinMul = 107u;
+constexpr unsigned kFibMul = 0x9E3779u;
+template <int T, int H>
+__device__ __forceinline__ unsigned hash_home(int key)
+{
+    static_assert(T >= 2 && T <= (1 << 24) && (T & (T - 1)) == 0, "table size");
+    if constexpr (H == kHashLin) {
+        return __umul24(static_cast<unsigned>(key), kLinMul) & (T - 1);
+    } else {
+        constexpr int kLog = __builtin_ctz(T);
+        return (__umul24(static_cast<unsigned>(key), kFibMul) & 0xFFFFFFu) >> (24 - kLog);
+    }
+}
 
 // Linear-probing insert of `key` (>= 0) into an int table of T (power of 2)
 // slots initialised to -1.  Returns the slot; `fresh` tells whether this
 // call created the key.  A plain LDS read comes first: most products of a
 // row repeat a key that is already in the table, and ds_read is cheaper than
 // ds_cmpst; the CAS is only issued on an empty slot.
-template <int T>
+template <int T, int H>
 __device__ __forceinline__ int hash_insert(int *tab, int key, bool &fresh)
 {
-    unsigned h = hash_slot(key) & (T - 1);
+    unsigned h = hash_home<T, H>(key);
     while (true) {
         int cur = tab[h];
         if (cur == -1) cur = atomicCAS(tab + h, -1, key);
@@ -318,30 +349,12 @@ __device__ __forceinline__ int xcd_tile()
 }
 __host__ __device__ constexpr int xcd_grid(int tiles) { return (tiles + kNumXcd - 1) / kNumXcd * kNumXcd; }
 
-// Same with a runtime power-of-two mask.
-__device__ __forceinline__ int hash_insert_mask(int *tab, int key, unsigned mask, bool &fresh)
-{
-    unsigned h = hash_slot(key) & mask;
-    while (true) {
-        const int old = atomicCAS(tab + h, -1, key);
-        if (old == -1) {
-            fresh = true;
-            return static_cast<int>(h);
-        }
-        if (old == key) {
-            fresh = false;
-            return static_cast<int>(h);
-        }
-        h = (h + 1) & mask;
-    }
-}
-
 // Lookup; returns -1 when the key is absent (the probe stops at an empty
 // slot or after T probes, so a mismatched pattern can never spin).
-template <int T>
+template <int T, int H>
 __device__ __forceinline__ int hash_find(const int *tab, int key)
 {
-    unsigned h = hash_slot(key) & (T - 1);
+    unsigned h = hash_home<T, H>(key);
     for (int i = 0; i < T; i++) {
         const int v = tab[h];
         if (v == key) return static_cast<int>(h);

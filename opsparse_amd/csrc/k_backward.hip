@@ -100,7 +100,7 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
         const long long c0 = crpt[r], c1 = crpt[r + 1];
         for (long long j = c0 + (tid % G); j < c1; j += G) {
             bool fresh;
-            const int h = hash_insert<T>(kt, ccol[j], fresh);
+            const int h = hash_insert<T, (G <= kWave ? kHashLin : kHashFib)>(kt, ccol[j], fresh);
             vt[h] = dc[j];
         }
     }
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
     if (ridx >= nrows) return;
     reduce_products<U, V>(arpt[r], arpt[r + 1], unit, NU, acol, aval, brpt, sda + (tid / U) * U, da,
                           [&](int k, V a) -> V {
-                              const int h = hash_find<T>(kt, bcol[k]);
+                              const int h = hash_find<T, (G <= kWave ? kHashLin : kHashFib)>(kt, bcol[k]);
                               const V g = h >= 0 ? vt[h] : V(0);
                               global_add(db + k, a * g);
                               return g * bval[k];

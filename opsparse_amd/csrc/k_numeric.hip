@@ -24,7 +24,7 @@ namespace {
 
 // Consumers of the product walk (device.hpp): accumulate a*b into the
 // row's hash table, or into a dense column window.
-template <int T, typename V>
+template <int T, int H, typename V>
 struct NumInsert {
     const int *__restrict__ bcol;
     const V *__restrict__ bval;
@@ -38,7 +38,7 @@ struct NumInsert {
     __device__ __forceinline__ void apply(const Item &it, V a)
     {
         bool fresh;
-        const int h = hash_insert<T>(kt, it.c, fresh);
+        const int h = hash_insert<T, H>(kt, it.c, fresh);
         atomicAdd(vt + h, a * it.b);
     }
 };
@@ -59,12 +59,14 @@ struct DenseInsert {
     {
         const int c = it.c - w0;
         atomicAdd(acc + c, a * it.b);
-        atomicOr(bm + (c >> 5), 1u << (c & 31));
+        const unsigned bit = 1u << (c & 31);
+        if (!(bm[c >> 5] & bit)) atomicOr(bm + (c >> 5), bit);
     }
 };
 
 // Bins 0-4: G <= 64 lanes per row, BS/G rows per block, register sort.
-template <int G, int T, int BS, typename V>
+// T table slots, at most NMAX entries per row (T >= 2 * NMAX).
+template <int G, int T, int NMAX, int BS, typename V>
 __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                   const V *__restrict__ aval, const int *__restrict__ brpt,
                                                   const int *__restrict__ bcol, const V *__restrict__ bval,
@@ -74,12 +76,12 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
 {
     constexpr int U = G;
     constexpr int RPB = BS / G;
-    constexpr int S = T / U;        // table slots per lane
-    constexpr int K = T / (2 * U);  // sorted entries per lane (n <= T/2)
-    static_assert(G <= kWave && BS % G == 0 && K >= 1, "bad config");
+    constexpr int S = T / U;     // table slots per lane
+    constexpr int K = NMAX / U;  // sorted entries per lane
+    static_assert(G <= kWave && BS % G == 0 && K >= 1 && T >= 2 * NMAX, "bad config");
     __shared__ __align__(16) int keys[RPB * T];
     __shared__ __align__(16) V vals[RPB * T];
-    __shared__ int ckeys[RPB * (T / 2)];
+    __shared__ int ckeys[RPB * NMAX];
 
     const int tid = threadIdx.x;
     const int slot = tid / G;
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     if (ridx < nrows) {
         r = rows ? rows[ridx] : ridx;
         const FullRow range{brpt};
-        NumInsert<T, V> op{bcol, bval, kt, vt};
+        NumInsert<T, kHashLin, V> op{bcol, bval, kt, vt};
         for_each_product<U, true, V>(arpt[r], arpt[r + 1], 0, 1, acol, aval, range, op);
     }
     __syncthreads();
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
 
     // Compact the row's keys (slot order) into its key list; the table stays
     // intact and supplies the values after the keys are sorted.
-    int *ck = ckeys + slot * (T / 2);
+    int *ck = ckeys + slot * NMAX;
     int n = 0;
     const unsigned long long below = gl ? ((1ull << gl) - 1ull) : 0ull;
 #pragma unroll
@@ -113,10 +115,10 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
         const bool valid = key != -1;
         const unsigned long long m = unit_ballot<U>(valid);
         const int pos = n + __popcll(m & below);
-        if (valid && pos < T / 2) ck[pos] = key;
+        if (valid && pos < NMAX) ck[pos] = key;
         n += __popcll(m);
     }
-    n = min(n, T / 2);
+    n = min(n, NMAX);
     wave_lds_sync();
 
     int sk[K];
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     for (int k = 0; k < K; k++) {
         const int j = k * U + gl;
         if (j < n_out) {
-            const int h = hash_find<T>(kt, sk[k]);
+            const int h = hash_find<T, kHashLin>(kt, sk[k]);
             ccol[off + j] = sk[k];
             cval[off + j] = h >= 0 ? vt[h] : V(0);
         }
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
         if (tid == 0) s_pos = 0;
         __syncthreads();
         const FullRow range{brpt};
-        NumInsert<T, V> op{bcol, bval, kt, vt};
+        NumInsert<T, kHashFib, V> op{bcol, bval, kt, vt};
         for_each_product<kWave, true, V>(arpt[r], arpt[r + 1], wave, NU, acol, aval, range, op);
         __syncthreads();
         const long long off = crpt[r];
@@ -191,7 +193,8 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
         for (int size = 2; size <= p2; size <<= 1) {
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
                 for (int i = tid; i < (p2 >> 1); i += BS) {
-                    const int lo = 2 * stride * (i / stride) + (i % stride);
+                    // pair index i -> (lo, lo + stride); stride is a power of two
+                    const int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));
                     const int hi = lo + stride;
                     const bool asc = (lo & size) == 0;
                     const int a = sk[lo], b = sk[hi];
@@ -205,7 +208,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
         }
         for (int j = tid; j < n; j += BS) {
             const int key = sk[j];
-            const int h = hash_find<T>(kt, key);
+            const int h = hash_find<T, kHashFib>(kt, key);
             ccol[off + j] = key;
             cval[off + j] = h >= 0 ? vt[h] : V(0);
         }
@@ -229,7 +232,8 @@ struct DenseWin {
 struct CursorRange {
     const int *__restrict__ brpt;
     const int *__restrict__ bcol;
-    int *__restrict__ cursor;
+    const int *__restrict__ cur_in;  // position after the previous window (unused on the first)
+    int *__restrict__ cur_out;       // position after this window
     int wend;
     bool first;
     bool last;  // wend covers all columns
@@ -237,7 +241,7 @@ struct CursorRange {
     __device__ __forceinline__ void operator()(int e, int k, int &bs, int &len) const
     {
         const int t = brpt[k + 1];
-        const int lo = first ? brpt[k] : cursor[e];
+        const int lo = first ? brpt[k] : cur_in[e];
         int hi;
         if (last) {
             hi = t;
@@ -259,7 +263,7 @@ struct CursorRange {
             }
             hi = l;
         }
-        cursor[e] = hi;
+        cur_out[e] = hi;
         if (hi < t) *next_col = min(*next_col, bcol[hi]);
         bs = lo;
         len = hi - lo;
@@ -268,14 +272,18 @@ struct CursorRange {
 
 // Bin 7: rows above 4096 entries.  Dense LDS accumulator + occupancy bitmap
 // per window of W columns; windows are written out in column order, so the
-// output is sorted without a sort.  cursor[] has one int per A-nonzero.
+// output is sorted without a sort.  cursor[] holds two ints per A-nonzero
+// index (span = highest A-nonzero index + 1): windows alternate which half
+// they read and which they write, so waves that share an A chunk (and all
+// compute the same cursor) never read a position another wave has already
+// advanced.
 template <typename V>
 __global__ __launch_bounds__(kLongBS) void k_num_long(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const V *__restrict__ aval, const int *__restrict__ brpt,
                                                       const int *__restrict__ bcol, const V *__restrict__ bval,
                                                       const int *__restrict__ rows, int nrows, int ncols,
                                                       const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                      V *__restrict__ cval, int *__restrict__ cursor)
+                                                      V *__restrict__ cval, int *__restrict__ cursor, int span)
 {
     using D = DenseWin<V>;
     extern __shared__ unsigned long long smem_u64[];
@@ -310,10 +318,13 @@ __global__ __launch_bounds__(kLongBS) void k_num_long(const int *__restrict__ ar
         __syncthreads();
         int w0 = mn == INT_MAX ? ncols : (mn / D::W) * D::W;
         bool first = true;
+        int parity = 0;
         while (w0 < ncols) {
             const int wend = min(ncols, w0 + D::W);
             int nxt = INT_MAX;
-            const CursorRange range{brpt, bcol, cursor, wend, first, wend >= ncols, &nxt};
+            int *cur_a = cursor + (parity ? span : 0);
+            int *cur_b = cursor + (parity ? 0 : span);
+            const CursorRange range{brpt, bcol, cur_a, cur_b, wend, first, wend >= ncols, &nxt};
             DenseInsert<V> op{bcol, bval, acc, bm, w0};
             for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op);
             nxt = unit_min<kWave>(nxt);
@@ -346,18 +357,19 @@ __global__ __launch_bounds__(kLongBS) void k_num_long(const int *__restrict__ ar
             for (int w = 0; w < NU; w++) nxt = min(nxt, s_min[w]);
             __syncthreads();
             first = false;
+            parity ^= 1;
             w0 = nxt == INT_MAX ? ncols : (nxt / D::W) * D::W;
         }
     }
 }
 
-template <int G, int T, int BS, typename V>
+template <int G, int T, int NMAX, int BS, typename V>
 void launch_small(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt, int *ccol,
                   V *cval, hipStream_t s)
 {
     constexpr int RPB = BS / G;
     const int grid = xcd_grid((nrows + RPB - 1) / RPB);
-    k_num_small<G, T, BS, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col,
+    k_num_small<G, T, NMAX, BS, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col,
                                                 static_cast<const V *>(B.val), rows, nrows, crpt, ccol, cval);
 }
 
@@ -397,28 +409,28 @@ void numeric_init_attributes()
 
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, int *cursor, hipStream_t s)
+                        int *ccol, V *cval, int *cursor, int span, hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
-        case 0: launch_small<16, 32, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
-        case 1: launch_small<32, 64, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
-        case 2: launch_small<64, 128, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
-        case 3: launch_small<64, 256, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
-        case 4: launch_small<64, 512, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 0: launch_small<16, 32, 16, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 1: launch_small<32, 64, 32, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 2: launch_small<64, 128, 64, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 3: launch_small<64, 256, 128, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 4: launch_small<64, 512, 256, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
         case 5: launch_block<2048, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
         case 6: launch_block<8192, 1024, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
         default:
             k_num_long<V><<<nrows, kLongBS, DenseWin<V>::lds, s>>>(
                 A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
-                B.N, crpt, ccol, cval, cursor);
+                B.N, crpt, ccol, cval, cursor, span);
             break;
     }
 }
 
 template void launch_numeric_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
-                                         double *, int *, hipStream_t);
+                                         double *, int *, int, hipStream_t);
 template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
-                                        float *, int *, hipStream_t);
+                                        float *, int *, int, hipStream_t);
 
 }  // namespace opsamd

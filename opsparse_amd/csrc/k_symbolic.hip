@@ -23,7 +23,7 @@ namespace {
 
 // Consumers of the product walk (device.hpp): insert B's column into the
 // row's hash table / bitmap and count the keys that were new.
-template <int T>
+template <int T, int H>
 struct SymInsert {
     const int *__restrict__ bcol;
     int *tab;
@@ -33,7 +33,7 @@ struct SymInsert {
     __device__ __forceinline__ void apply(int c, double)
     {
         bool fresh;
-        hash_insert<T>(tab, c, fresh);
+        hash_insert<T, H>(tab, c, fresh);
         cnt += fresh ? 1 : 0;
     }
 };
@@ -49,6 +49,9 @@ struct BitmapInsert {
     {
         const int c = col - w0;
         const unsigned bit = 1u << (c & 31);
+        // Popular columns are set early; a plain read skips the atomic (and
+        // its same-address serialisation) for them.
+        if (bm[c >> 5] & bit) return;
         const unsigned old = atomicOr(&bm[c >> 5], bit);
         cnt += (old & bit) ? 0 : 1;
     }
@@ -80,7 +83,8 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     int r = 0;
     if (ridx < nrows) {
         r = rows ? rows[ridx] : ridx;
-        SymInsert<T> op{bcol, tab + slot * T};
+        // Short-row bins hash with the linear family, wide rows with Fibonacci.
+        SymInsert<T, (G <= kWave ? kHashLin : kHashFib)> op{bcol, tab + slot * T};
         const FullRow range{brpt};
         for_each_product<U, false, double>(arpt[r], arpt[r + 1], unit, NU, acol, static_cast<const double *>(nullptr),
                                            range, op);

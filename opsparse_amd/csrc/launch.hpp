@@ -78,7 +78,7 @@ size_t symbolic_long_lds();
 // --- numeric (k_numeric.hip) ---
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, int *cursor, hipStream_t s);
+                        int *ccol, V *cval, int *cursor, int span, hipStream_t s);
 // One-time kernel attribute setup (dynamic LDS limits).
 void numeric_init_attributes();
 void symbolic_init_attributes();

@@ -95,7 +95,8 @@ struct PlanCore {
     long long nnz = 0;
     long long *crpt64 = nullptr;  // M+1, device
     int *bins = nullptr;          // numeric bin lists (nullptr: identity over one bin)
-    int *cursor = nullptr;        // one int per A-nonzero, long-row bin only
+    int *cursor = nullptr;        // two ints per A-nonzero index, long-row bin only
+    int span = 0;                 // highest A-nonzero index + 1
     int num_size[kNumBins] = {};
     int num_off[kNumBins] = {};
     DevStats stats{};
@@ -294,7 +295,7 @@ void numeric_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, const Plan
     const bool ev = t && t->kernel_events;
     dispatch_bins(ws, s, pc.num_size, ev, [&](int b, hipStream_t ps) {
         launch_numeric_bin<V>(b, A, B, pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b], pc.crpt64, ccol,
-                              cval, pc.cursor, ps);
+                              cval, pc.cursor, pc.span, ps);
     });
 }
 
@@ -397,7 +398,8 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
         C32->val = cval;
     }
     if (pc.num_size[kNumBins - 1] > 0) {
-        ws->cursor.ensure(static_cast<size_t>(a_index_span(A, s)) + 1);
+        pc.span = static_cast<int>(a_index_span(A, s));
+        ws->cursor.ensure(2 * static_cast<size_t>(pc.span) + 2);
         pc.cursor = ws->cursor.p;
     }
     if (t) t->allocate = now_s() - t0;
@@ -644,7 +646,8 @@ int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_p
         p->bins.ensure(static_cast<size_t>(M) + 1);
         symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr);
         if (p->core.num_size[kNumBins - 1] > 0) {
-            p->cursor.ensure(static_cast<size_t>(a_index_span(A, s)) + 1);
+            p->core.span = static_cast<int>(a_index_span(A, s));
+            p->cursor.ensure(2 * static_cast<size_t>(p->core.span) + 2);
             p->core.cursor = p->cursor.p;
         }
         CK(hipStreamSynchronize(s));

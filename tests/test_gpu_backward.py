@@ -1,0 +1,95 @@
+"""GPU parity of the masked backward (dA on pattern(A), dB on pattern(B)).
+
+Checked against the FP64 CPU oracle gradient with the error normalised by
+sum |terms| (the summation-order-independent bound; computed by running the
+oracle on absolute values), and against FP64 central differences of the
+bilinear loss L = <dC, A*B> (exact up to rounding for any step).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import opsparse_amd.gen as G
+from opsparse_amd.spgemm import DeviceCSR, Plan, SpGEMMFunction, backward
+from helpers import random_csr
+
+pytestmark = pytest.mark.gpu
+
+TOL = {torch.float64: 1e-12, torch.float32: 1e-5}
+
+
+def absv(X):
+    return G.HostCSR(X.M, X.N, X.rpt, X.col, np.abs(X.val))
+
+
+def check_grad(A, B, dtype, seed=0):
+    rng = np.random.default_rng(seed)
+    dA, dB = DeviceCSR.from_host(A, dtype), DeviceCSR.from_host(B, dtype)
+    plan = Plan(dA, dB)
+    rpt = plan.rowptr64()
+    col, _ = plan.numeric(dA, dB)
+    dc = rng.normal(size=plan.nnz)
+    ga, gb = backward(dA, dB, rpt, col, torch.from_numpy(dc).to(device="cuda", dtype=dtype))
+    torch.cuda.synchronize()
+    rpt_h, col_h = rpt.cpu().numpy(), col.cpu().numpy()
+    if dtype == torch.float32:
+        A, B = A.astype(np.float32), B.astype(np.float32)
+        dc = dc.astype(np.float32)
+    ra, rb = oracle.backward(A, B, rpt_h, col_h, dc)
+    sa, sb = oracle.backward(absv(A), absv(B), rpt_h, col_h, np.abs(dc))
+    ea = np.abs(ga.double().cpu().numpy() - ra) / np.maximum(sa, 1e-300)
+    eb = np.abs(gb.double().cpu().numpy() - rb) / np.maximum(sb, 1e-300)
+    err = max(ea.max(initial=0.0), eb.max(initial=0.0))
+    assert err <= TOL[dtype], f"gradient error {err:.3e}"
+    return err
+
+
+def test_backward_rmat_fp64_fp32(gpu):
+    A = G.rmat(11, 8, 21, normal=True)
+    B = G.rmat(11, 8, 22, normal=True)
+    check_grad(A, B, torch.float64)
+    check_grad(A, B, torch.float32)
+
+
+def test_backward_stencil_and_long_rows(gpu):
+    S = G.stencil27(12, 3)
+    check_grad(S, S, torch.float64)
+    rng = np.random.default_rng(5)
+    N = 20000
+    A = random_csr(24, N, np.r_[rng.integers(100, 160, 12), rng.integers(1, 4, 12)], rng)
+    B = random_csr(N, N, rng.integers(50, 90, N), rng)
+    check_grad(A, B, torch.float64)  # rows of C above 4096 entries use the sorted-row search
+
+
+def test_autograd_function_and_finite_differences(gpu):
+    A = G.rmat(9, 6, 31, normal=True)
+    B = G.rmat(9, 6, 32, normal=True)
+    dA, dB = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+    plan = Plan(dA, dB)
+    a = dA.val.clone().requires_grad_(True)
+    b = dB.val.clone().requires_grad_(True)
+    val, col = SpGEMMFunction.apply(a, b, dA, dB, plan)
+    w = torch.randn(val.shape, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+    loss = (val * w).sum()
+    loss.backward()
+    ra, rb = oracle.backward(A, B, plan.rowptr64().cpu().numpy(), col.cpu().numpy(), w.cpu().numpy())
+    assert np.allclose(a.grad.cpu().numpy(), ra, rtol=1e-11, atol=1e-11)
+    assert np.allclose(b.grad.cpu().numpy(), rb, rtol=1e-11, atol=1e-11)
+
+    def L(av, bv):
+        _, v = plan.numeric(dA.with_values(av), dB.with_values(bv))
+        return float((v * w).sum())
+
+    h = 1e-3
+    rng = np.random.default_rng(0)
+    for e in rng.choice(A.nnz, 16, replace=False):
+        d = torch.zeros_like(dA.val)
+        d[e] = h
+        fd = (L(dA.val + d, dB.val) - L(dA.val - d, dB.val)) / (2 * h)
+        assert abs(fd - a.grad[e].item()) <= 1e-9 * (1 + abs(fd))
+    for e in rng.choice(B.nnz, 16, replace=False):
+        d = torch.zeros_like(dB.val)
+        d[e] = h
+        fd = (L(dA.val, dB.val + d) - L(dA.val, dB.val - d)) / (2 * h)
+        assert abs(fd - b.grad[e].item()) <= 1e-9 * (1 + abs(fd))
