@@ -169,6 +169,40 @@ int ops_plan_numeric(ops_plan *plan, ops_dtype dt, const ops_csr *A, const ops_c
 int ops_plan_numeric_graph(ops_plan *plan, ops_dtype dt, const ops_csr *A, const ops_csr *B, int32_t *ccol,
                            void *cval, void *stream);
 
+/* ---- Tuning: bin boundaries and the short-row hash family ---------------- */
+
+/* The reference hard-codes its bin bounds (kernels/binning.cuh:21, 51, 105,
+ * 143) and hash (utils/config.hpp:32); here they are run-time settings of a
+ * workspace, chosen by the autotuner (opsparse_amd/autotune.py).  A bound may
+ * not exceed the capacity of its bin's kernel (ops_tuning_caps) and bounds
+ * must not decrease from bin to bin; bin 7 is unbounded. */
+typedef struct ops_tuning {
+    int32_t sym_bound[OPS_NUM_BINS - 1]; /* inclusive upper bound of row products per symbolic bin */
+    int32_t num_bound[OPS_NUM_BINS - 1]; /* inclusive upper bound of row nnz per numeric bin */
+    int32_t short_hash;                  /* bins 0-4: 0 = linear (col * 107 mod T), 1 = Fibonacci */
+} ops_tuning;
+
+int ops_tuning_default(ops_tuning *t);
+int ops_tuning_caps(ops_tuning *caps);
+/* Validates and installs; later calls and plans on this workspace use it. */
+int ops_workspace_set_tuning(ops_workspace *ws, const ops_tuning *t);
+int ops_workspace_get_tuning(const ops_workspace *ws, ops_tuning *t);
+
+/* ---- Batches of independent products (no counterpart in the reference) --- */
+
+/* Packs `count` device CSR matrices block-diagonally into one library-
+ * allocated CSR (free with ops_csr_free): instance i occupies rows
+ * [row_base[i], row_base[i+1]) and columns [col_base[i], col_base[i+1]).
+ * Packing the A_i and the B_i of independent products C_i = A_i * B_i gives
+ * packed A and B whose product is the block-diagonal of the C_i, so one plan
+ * (one setup, one set of per-bin kernels, bins shared across instances)
+ * computes every C_i; C_i is rows [rowA_base[i], rowA_base[i+1]) of the
+ * packed C with columns shifted by colB_base[i].  `instances` is a host
+ * array; each descriptor's nnz must equal rpt[M] - rpt[0].  row_base and
+ * col_base (count + 1 entries each, host) may be NULL. */
+int ops_batch_pack(int32_t count, const ops_csr *instances, ops_dtype dt, ops_csr *packed, int64_t *row_base,
+                   int64_t *col_base, void *stream);
+
 /* ---- Masked backward at fixed sparsity (no counterpart in the reference) -- */
 
 /* Given dC on pattern(C) (ccol sorted per row, row offsets crpt64),

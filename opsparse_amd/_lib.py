@@ -56,6 +56,24 @@ class OpsTimings(C.Structure):
                 ("num_ms", C.c_double * NUM_BINS)]
 
 
+class OpsTuning(C.Structure):
+    _fields_ = [("sym_bound", C.c_int32 * (NUM_BINS - 1)), ("num_bound", C.c_int32 * (NUM_BINS - 1)),
+                ("short_hash", C.c_int32)]
+
+    def as_dict(self):
+        return {"sym_bound": list(self.sym_bound), "num_bound": list(self.num_bound), "short_hash": self.short_hash}
+
+    @staticmethod
+    def from_dict(d):
+        t = OpsTuning()
+        for i, v in enumerate(d["sym_bound"]):
+            t.sym_bound[i] = v
+        for i, v in enumerate(d["num_bound"]):
+            t.num_bound[i] = v
+        t.short_hash = d["short_hash"]
+        return t
+
+
 class OpsHostCsr(C.Structure):
     _fields_ = [("M", C.c_int32), ("N", C.c_int32), ("nnz", C.c_int64), ("rpt", C.POINTER(C.c_int32)),
                 ("col", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double))]
@@ -86,6 +104,11 @@ SIGNATURES = {
     "ops_plan_numeric": (C.c_int, [_vp, C.c_int, _P(OpsCsr), _P(OpsCsr), _vp, _vp, _vp]),
     "ops_plan_numeric_graph": (C.c_int, [_vp, C.c_int, _P(OpsCsr), _P(OpsCsr), _vp, _vp, _vp]),
     "ops_spgemm_backward": (C.c_int, [_vp, C.c_int, _P(OpsCsr), _P(OpsCsr), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ops_tuning_default": (C.c_int, [_P(OpsTuning)]),
+    "ops_tuning_caps": (C.c_int, [_P(OpsTuning)]),
+    "ops_workspace_set_tuning": (C.c_int, [_vp, _P(OpsTuning)]),
+    "ops_workspace_get_tuning": (C.c_int, [_vp, _P(OpsTuning)]),
+    "ops_batch_pack": (C.c_int, [C.c_int32, _P(OpsCsr), C.c_int, _P(OpsCsr), _P(C.c_int64), _P(C.c_int64), _vp]),
     "ops_gen_uniform_rows": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _P(OpsHostCsr)]),
     "ops_gen_stencil27": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                                     C.c_uint64, _P(OpsHostCsr)]),

@@ -19,6 +19,8 @@
 #include <climits>
 #include <cstdint>
 
+#include "launch.hpp"
+
 namespace opsamd {
 
 constexpr int kWave = 64;
@@ -298,7 +300,6 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
 //    bits; power-law (R-MAT) column indices have strongly skewed low bits,
 //    which piles keys onto a few home slots under LIN.  Used for the long-row
 //    bins, where power-law rows put most of their products.
-enum HashKind { kHashLin = 0, kHashFib = 1 };
 constexpr unsigned kLinMul = 107u;
 constexpr unsigned kFibMul = 0x9E3779u;
 template <int T, int H>

@@ -66,7 +66,7 @@ struct DenseInsert {
 
 // Bins 0-4: G <= 64 lanes per row, BS/G rows per block, register sort.
 // T table slots, at most NMAX entries per row (T >= 2 * NMAX).
-template <int G, int T, int NMAX, int BS, typename V>
+template <int G, int T, int NMAX, int BS, int H, typename V>
 __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                   const V *__restrict__ aval, const int *__restrict__ brpt,
                                                   const int *__restrict__ bcol, const V *__restrict__ bval,
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     if (ridx < nrows) {
         r = rows ? rows[ridx] : ridx;
         const FullRow range{brpt};
-        NumInsert<T, kHashLin, V> op{bcol, bval, kt, vt};
+        NumInsert<T, H, V> op{bcol, bval, kt, vt};
         for_each_product<U, true, V>(arpt[r], arpt[r + 1], 0, 1, acol, aval, range, op);
     }
     __syncthreads();
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     for (int k = 0; k < K; k++) {
         const int j = k * U + gl;
         if (j < n_out) {
-            const int h = hash_find<T, kHashLin>(kt, sk[k]);
+            const int h = hash_find<T, H>(kt, sk[k]);
             ccol[off + j] = sk[k];
             cval[off + j] = h >= 0 ? vt[h] : V(0);
         }
@@ -365,12 +365,18 @@ __global__ __launch_bounds__(kLongBS) void k_num_long(const int *__restrict__ ar
 
 template <int G, int T, int NMAX, int BS, typename V>
 void launch_small(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt, int *ccol,
-                  V *cval, hipStream_t s)
+                  V *cval, int hash, hipStream_t s)
 {
     constexpr int RPB = BS / G;
     const int grid = xcd_grid((nrows + RPB - 1) / RPB);
-    k_num_small<G, T, NMAX, BS, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col,
-                                                static_cast<const V *>(B.val), rows, nrows, crpt, ccol, cval);
+    if (hash == kHashLin)
+        k_num_small<G, T, NMAX, BS, kHashLin, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val),
+                                                                     B.rpt, B.col, static_cast<const V *>(B.val), rows,
+                                                                     nrows, crpt, ccol, cval);
+    else
+        k_num_small<G, T, NMAX, BS, kHashFib, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val),
+                                                                     B.rpt, B.col, static_cast<const V *>(B.val), rows,
+                                                                     nrows, crpt, ccol, cval);
 }
 
 template <int T, int BS, typename V>
@@ -409,15 +415,15 @@ void numeric_init_attributes()
 
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, int *cursor, int span, hipStream_t s)
+                        int *ccol, V *cval, int *cursor, int span, int hash, hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
-        case 0: launch_small<16, 32, 16, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
-        case 1: launch_small<32, 64, 32, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
-        case 2: launch_small<64, 128, 64, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
-        case 3: launch_small<64, 256, 128, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
-        case 4: launch_small<64, 512, 256, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 0: launch_small<16, 32, 16, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
+        case 1: launch_small<32, 64, 32, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
+        case 2: launch_small<64, 128, 64, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
+        case 3: launch_small<64, 256, 128, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
+        case 4: launch_small<64, 512, 256, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
         case 5: launch_block<2048, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
         case 6: launch_block<8192, 1024, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
         default:
@@ -429,8 +435,8 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows
 }
 
 template void launch_numeric_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
-                                         double *, int *, int, hipStream_t);
+                                         double *, int *, int, int, hipStream_t);
 template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
-                                        float *, int *, int, hipStream_t);
+                                        float *, int *, int, int, hipStream_t);
 
 }  // namespace opsamd

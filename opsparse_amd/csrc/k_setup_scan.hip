@@ -14,19 +14,19 @@ namespace opsamd {
 
 namespace {
 
-__device__ __forceinline__ int dev_sym_bin(int f)
+__device__ __forceinline__ int dev_sym_bin(int f, const BinBounds &bb)
 {
     int b = 0;
 #pragma unroll
-    for (int i = 0; i < kNumBins - 1; i++) b += f > kSymBound[i] ? 1 : 0;
+    for (int i = 0; i < kNumBins - 1; i++) b += f > bb.sym[i] ? 1 : 0;
     return b;
 }
 
-__device__ __forceinline__ int dev_num_bin(int n)
+__device__ __forceinline__ int dev_num_bin(int n, const BinBounds &bb)
 {
     int b = 0;
 #pragma unroll
-    for (int i = 0; i < kNumBins - 1; i++) b += n > kNumBound[i] ? 1 : 0;
+    for (int i = 0; i < kNumBins - 1; i++) b += n > bb.num[i] ? 1 : 0;
     return b;
 }
 
@@ -49,7 +49,8 @@ constexpr int kLongRowA = 256;
 
 __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const int *__restrict__ brpt, int M, int *__restrict__ row_flop,
-                                                      int *__restrict__ row_nnz, DevStats *__restrict__ st)
+                                                      int *__restrict__ row_nnz, DevStats *__restrict__ st,
+                                                      const BinBounds bb)
 {
     __shared__ int s_hist[kNumBins];
     __shared__ unsigned long long s_bflop[kNumBins];
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
             if (fi == 0) {
                 row_nnz[r] = 0;
             } else {
-                const int b = dev_sym_bin(fi);
+                const int b = dev_sym_bin(fi, bb);
                 atomicAdd(&s_hist[b], 1);
                 atomicAdd(&s_bflop[b], f);
                 atomicAdd(&s_bannz[b], static_cast<unsigned long long>(arpt[r + 1] - arpt[r]));
@@ -157,7 +158,8 @@ constexpr int kScatBS = 1024;
 constexpr int kScatItems = 4;
 
 __global__ __launch_bounds__(kScatBS) void k_bin_scatter(const int *__restrict__ key, int M, int which,
-                                                         DevStats *__restrict__ st, int *__restrict__ bins)
+                                                         DevStats *__restrict__ st, int *__restrict__ bins,
+                                                         const BinBounds bb)
 {
     __shared__ int s_cnt[kNumBins];
     __shared__ int s_base[kNumBins];
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(kScatBS) void k_bin_scatter(const int *__restrict__
         rank[i] = 0;
         if (r < M) {
             const int v = key[r];
-            if (v > 0) bin[i] = which ? dev_num_bin(v) : dev_sym_bin(v);
+            if (v > 0) bin[i] = which ? dev_num_bin(v, bb) : dev_sym_bin(v, bb);
         }
 #pragma unroll
         for (int b = 0; b < kNumBins; b++) {
@@ -210,7 +212,8 @@ constexpr int kScanTile = kScanBS * kScanItems;
 __global__ __launch_bounds__(kScanBS) void k_scan_reduce(const int *__restrict__ row_nnz,
                                                          const int *__restrict__ row_flop,
                                                          const int *__restrict__ arpt, int M,
-                                                         long long *__restrict__ block_sums, DevStats *__restrict__ st)
+                                                         long long *__restrict__ block_sums, DevStats *__restrict__ st,
+                                                         const BinBounds bb)
 {
     __shared__ unsigned long long s_red[kScanBS / kWave];
     __shared__ int s_hist[kNumBins];
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(kScanBS) void k_scan_reduce(const int *__restrict__
             sum += static_cast<unsigned long long>(n);
             mx = max(mx, n);
             if (n > 0) {
-                const int b = dev_num_bin(n);
+                const int b = dev_num_bin(n, bb);
                 atomicAdd(&s_hist[b], 1);
                 atomicAdd(&s_f[b], static_cast<unsigned long long>(row_flop[r]));
                 atomicAdd(&s_n[b], static_cast<unsigned long long>(n));
@@ -374,27 +377,29 @@ void launch_rownnz_from_rpt(const long long *crpt, int M, int *row_nnz, hipStrea
 
 void launch_zero_stats(DevStats *st, hipStream_t s) { k_zero_stats<<<1, 256, 0, s>>>(st); }
 
-void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, hipStream_t s)
+void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, const BinBounds &bb,
+                  hipStream_t s)
 {
     if (A.M == 0) return;
     const int grid = std::min((A.M + kSetupRows - 1) / kSetupRows, kSetupMaxGrid);
-    k_setup<<<grid, kSetupRows, 0, s>>>(A.rpt, A.col, B.rpt, A.M, row_flop, row_nnz, st);
+    k_setup<<<grid, kSetupRows, 0, s>>>(A.rpt, A.col, B.rpt, A.M, row_flop, row_nnz, st, bb);
 }
 
-void launch_bin_scatter(const int *key, int M, int which, DevStats *st, int *bins, hipStream_t s)
+void launch_bin_scatter(const int *key, int M, int which, DevStats *st, int *bins, const BinBounds &bb,
+                        hipStream_t s)
 {
     if (M == 0) return;
     constexpr int tile = kScatBS * kScatItems;
-    k_bin_scatter<<<(M + tile - 1) / tile, kScatBS, 0, s>>>(key, M, which, st, bins);
+    k_bin_scatter<<<(M + tile - 1) / tile, kScatBS, 0, s>>>(key, M, which, st, bins, bb);
 }
 
 int scan_blocks(int M) { return (M + kScanTile - 1) / kScanTile; }
 
 void launch_scan(const int *row_nnz, const int *row_flop, const int *arpt, int M, long long *block_sums,
-                 long long *crpt64, int *crpt32, DevStats *st, hipStream_t s)
+                 long long *crpt64, int *crpt32, DevStats *st, const BinBounds &bb, hipStream_t s)
 {
     const int nb = scan_blocks(M);
-    if (nb > 0) k_scan_reduce<<<nb, kScanBS, 0, s>>>(row_nnz, row_flop, arpt, M, block_sums, st);
+    if (nb > 0) k_scan_reduce<<<nb, kScanBS, 0, s>>>(row_nnz, row_flop, arpt, M, block_sums, st, bb);
     k_scan_blocks<<<1, kScanBS, 0, s>>>(block_sums, nb, st);
     k_scan_write<<<nb > 0 ? nb : 1, kScanBS, 0, s>>>(row_nnz, M, block_sums, nb, crpt64, crpt32);
 }

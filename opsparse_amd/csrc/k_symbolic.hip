@@ -59,7 +59,7 @@ struct BitmapInsert {
 
 // G lanes per row (G <= 64: BS/G rows per block; G == BS > 64: one row per
 // block, G/64 waves share the table).
-template <int G, int T, int BS>
+template <int G, int T, int BS, int H>
 __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                  const int *__restrict__ brpt, const int *__restrict__ bcol,
                                                  const int *__restrict__ rows, int nrows, int *__restrict__ row_nnz)
@@ -83,8 +83,7 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     int r = 0;
     if (ridx < nrows) {
         r = rows ? rows[ridx] : ridx;
-        // Short-row bins hash with the linear family, wide rows with Fibonacci.
-        SymInsert<T, (G <= kWave ? kHashLin : kHashFib)> op{bcol, tab + slot * T};
+        SymInsert<T, H> op{bcol, tab + slot * T};
         const FullRow range{brpt};
         for_each_product<U, false, double>(arpt[r], arpt[r + 1], unit, NU, acol, static_cast<const double *>(nullptr),
                                            range, op);
@@ -173,12 +172,20 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
     }
 }
 
+// Short-row bins (G <= 64) take the hash family chosen by the tuning; wide
+// rows (bins 5, 6) always hash with the Fibonacci family.
 template <int G, int T, int BS>
-void launch_hash(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, hipStream_t s)
+void launch_hash(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash, hipStream_t s)
 {
     constexpr int RPB = BS / G;
     const int grid = xcd_grid((nrows + RPB - 1) / RPB);
-    k_sym_hash<G, T, BS><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+    if constexpr (G <= kWave) {
+        if (hash == kHashLin) {
+            k_sym_hash<G, T, BS, kHashLin><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+            return;
+        }
+    }
+    k_sym_hash<G, T, BS, kHashFib><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
 }
 
 }  // namespace
@@ -191,18 +198,18 @@ void symbolic_init_attributes()
                               static_cast<int>(symbolic_long_lds()));
 }
 
-void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz,
+void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash,
                          hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
-        case 0: launch_hash<16, 32, 256>(A, B, rows, nrows, row_nnz, s); break;
-        case 1: launch_hash<32, 64, 256>(A, B, rows, nrows, row_nnz, s); break;
-        case 2: launch_hash<64, 128, 256>(A, B, rows, nrows, row_nnz, s); break;
-        case 3: launch_hash<64, 512, 256>(A, B, rows, nrows, row_nnz, s); break;
-        case 4: launch_hash<64, 1024, 256>(A, B, rows, nrows, row_nnz, s); break;
-        case 5: launch_hash<256, 4096, 256>(A, B, rows, nrows, row_nnz, s); break;
-        case 6: launch_hash<1024, 16384, 1024>(A, B, rows, nrows, row_nnz, s); break;
+        case 0: launch_hash<16, 32, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
+        case 1: launch_hash<32, 64, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
+        case 2: launch_hash<64, 128, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
+        case 3: launch_hash<64, 512, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
+        case 4: launch_hash<64, 1024, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
+        case 5: launch_hash<256, 4096, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
+        case 6: launch_hash<1024, 16384, 1024>(A, B, rows, nrows, row_nnz, hash, s); break;
         default: {
             // Bitmap sized to the column count (up to 2^20 columns per window),
             // so narrow products keep more blocks resident per CU.

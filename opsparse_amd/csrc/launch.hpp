@@ -11,6 +11,9 @@ namespace opsamd {
 
 constexpr int kNumBins = 8;
 
+// Hash families for the LDS tables (see hash_home in device.hpp).
+enum HashKind { kHashLin = 0, kHashFib = 1 };
+
 // Symbolic bins by row products f (inclusive upper bounds); rows with f == 0
 // are not binned (their nnz is 0).  Bin 7 is the long-row path.  The
 // reference bins by the same quantity with bounds {26, 426, ..., 10240, inf}
@@ -29,6 +32,30 @@ constexpr int kSymBound[kNumBins] = {16, 32, 64, 256, 1024, 4096, 16384, INT_MAX
 //   T:    32   64   128  256   512   2048   8192   dense column windows
 //   sort: unit bitonic in registers (bins 0-4), LDS bitonic (5-6), none (7)
 constexpr int kNumBound[kNumBins] = {16, 32, 64, 128, 256, 1024, 4096, INT_MAX};
+
+// Largest allowed bound per bin: a symbolic bin-b kernel's table holds T_b
+// keys, so rows with at most T_b products can never overflow it; a numeric
+// bin-b kernel holds at most NMAX_b entries per row.
+constexpr int kSymCap[kNumBins - 1] = {32, 64, 128, 512, 1024, 4096, 16384};
+constexpr int kNumCap[kNumBins - 1] = {16, 32, 64, 128, 256, 1024, 4096};
+
+// Run-time bin boundaries (inclusive upper bounds of bins 0..6; bin 7 is
+// unbounded).  Defaults are kSymBound / kNumBound; the autotuner may lower
+// them (moving rows to wider kernels with larger tables) within the caps.
+struct BinBounds {
+    int sym[kNumBins - 1];
+    int num[kNumBins - 1];
+};
+
+inline BinBounds default_bounds()
+{
+    BinBounds b{};
+    for (int i = 0; i < kNumBins - 1; i++) {
+        b.sym[i] = kSymBound[i];
+        b.num[i] = kNumBound[i];
+    }
+    return b;
+}
 
 // Device-side counters of one call (zeroed by k_zero_stats).
 struct DevStats {
@@ -57,20 +84,23 @@ struct CsrIn {
 // --- setup / binning / scan (k_setup_scan.hip) ---
 void launch_zero_stats(DevStats *st, hipStream_t s);
 // row_flop[M] (int32, saturated), per-bin counts; rows with f == 0 get row_nnz = 0.
-void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, hipStream_t s);
+void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, const BinBounds &bb,
+                  hipStream_t s);
 // Scatters row ids into bins (which = 0: symbolic by row_flop, 1: numeric by row_nnz).
-void launch_bin_scatter(const int *key, int M, int which, DevStats *st, int *bins, hipStream_t s);
+void launch_bin_scatter(const int *key, int M, int which, DevStats *st, int *bins, const BinBounds &bb,
+                        hipStream_t s);
 // Exclusive scan of row_nnz into crpt64 (and crpt32 when nnz(C) fits int32
 // and crpt32 != nullptr); numeric bin statistics.  block_sums needs
 // scan_blocks(M) + 1 int64 entries.
 int scan_blocks(int M);
 void launch_scan(const int *row_nnz, const int *row_flop, const int *arpt, int M, long long *block_sums,
-                 long long *crpt64, int *crpt32, DevStats *st, hipStream_t s);
+                 long long *crpt64, int *crpt32, DevStats *st, const BinBounds &bb, hipStream_t s);
 void launch_narrow_rpt(const long long *crpt64, int M, int *crpt32, hipStream_t s);
 
 // --- symbolic (k_symbolic.hip) ---
 // Computes row_nnz for the rows of one bin.  rows == nullptr means rows 0..nrows-1.
-void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz,
+// hash: family for the short-row bins 0-4 (kHashLin or kHashFib in device.hpp).
+void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash,
                          hipStream_t s);
 // Bytes of dynamic LDS the long-row symbolic kernel asks for.
 size_t symbolic_long_lds();
@@ -78,7 +108,7 @@ size_t symbolic_long_lds();
 // --- numeric (k_numeric.hip) ---
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, int *cursor, int span, hipStream_t s);
+                        int *ccol, V *cval, int *cursor, int span, int hash, hipStream_t s);
 // One-time kernel attribute setup (dynamic LDS limits).
 void numeric_init_attributes();
 void symbolic_init_attributes();
@@ -92,18 +122,5 @@ void launch_zero_values(void *p, long long n, bool f32, hipStream_t s);
 // row_nnz[i] = crpt[i+1] - crpt[i] (numeric binning key for the backward).
 void launch_rownnz_from_rpt(const long long *crpt, int M, int *row_nnz, hipStream_t s);
 
-inline int sym_bin_of(long long f)
-{
-    for (int b = 0; b < kNumBins; b++)
-        if (f <= kSymBound[b]) return b;
-    return kNumBins - 1;
-}
-
-inline int num_bin_of(long long n)
-{
-    for (int b = 0; b < kNumBins; b++)
-        if (n <= kNumBound[b]) return b;
-    return kNumBins - 1;
-}
 
 }  // namespace opsamd

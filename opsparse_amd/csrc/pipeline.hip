@@ -86,6 +86,8 @@ struct ops_workspace {
     DevStats *h_stats = nullptr;  // pinned
     DevBuf<int> row_flop, row_nnz, bins, cursor, crpt32_tmp;
     DevBuf<long long> block_sums, crpt64;
+    BinBounds bounds = default_bounds();
+    int short_hash = kHashLin;
     std::string err;
 };
 
@@ -99,6 +101,7 @@ struct PlanCore {
     int span = 0;                 // highest A-nonzero index + 1
     int num_size[kNumBins] = {};
     int num_off[kNumBins] = {};
+    int hash = kHashLin;  // hash family of the short-row bins
     DevStats stats{};
 };
 
@@ -187,13 +190,15 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     pc.M = M;
     pc.N = B.N;
     pc.crpt64 = crpt64_dst;
+    pc.hash = ws->short_hash;
+    const BinBounds &bb = ws->bounds;
     double t0 = now_s();
 
     ws->row_flop.ensure(static_cast<size_t>(M) + 1);
     ws->row_nnz.ensure(static_cast<size_t>(M) + 1);
     ws->block_sums.ensure(static_cast<size_t>(scan_blocks(M)) + 2);
     launch_zero_stats(ws->d_stats, s);
-    launch_setup(A, B, ws->row_flop.p, ws->row_nnz.p, ws->d_stats, s);
+    launch_setup(A, B, ws->row_flop.p, ws->row_nnz.p, ws->d_stats, bb, s);
     CK(hipGetLastError());
     CK(hipMemcpyAsync(ws->h_stats, ws->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
@@ -224,7 +229,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
             }
         }
         if (!(nonempty == 1 && st.sym_size[only] == M) && nonempty > 0) {
-            launch_bin_scatter(ws->row_flop.p, M, 0, ws->d_stats, bins_dst, s);
+            launch_bin_scatter(ws->row_flop.p, M, 0, ws->d_stats, bins_dst, bb, s);
             CK(hipGetLastError());
             sym_rows = bins_dst;
         }
@@ -236,7 +241,8 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
 
     t0 = now_s();
     dispatch_bins(ws, s, st.sym_size, ev, [&](int b, hipStream_t ps) {
-        launch_symbolic_bin(b, A, B, sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b], ws->row_nnz.p, ps);
+        launch_symbolic_bin(b, A, B, sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b], ws->row_nnz.p,
+                            pc.hash, ps);
     });
     if (sep) {
         CK(hipStreamSynchronize(s));
@@ -245,7 +251,8 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     if (ev) read_kernel_ms(ws, st.sym_size, t->sym_ms);
 
     t0 = now_s();
-    launch_scan(ws->row_nnz.p, ws->row_flop.p, A.rpt, M, ws->block_sums.p, crpt64_dst, crpt32_dst, ws->d_stats, s);
+    launch_scan(ws->row_nnz.p, ws->row_flop.p, A.rpt, M, ws->block_sums.p, crpt64_dst, crpt32_dst, ws->d_stats, bb,
+                s);
     CK(hipGetLastError());
     CK(hipMemcpyAsync(ws->h_stats, ws->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
@@ -278,7 +285,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     }
     pc.bins = nullptr;
     if (!(nonempty == 1 && st.num_size[only] == M) && nonempty > 0) {
-        launch_bin_scatter(ws->row_nnz.p, M, 1, ws->d_stats, bins_dst, s);
+        launch_bin_scatter(ws->row_nnz.p, M, 1, ws->d_stats, bins_dst, bb, s);
         CK(hipGetLastError());
         pc.bins = bins_dst;
     }
@@ -295,7 +302,7 @@ void numeric_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, const Plan
     const bool ev = t && t->kernel_events;
     dispatch_bins(ws, s, pc.num_size, ev, [&](int b, hipStream_t ps) {
         launch_numeric_bin<V>(b, A, B, pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b], pc.crpt64, ccol,
-                              cval, pc.cursor, pc.span, ps);
+                              cval, pc.cursor, pc.span, pc.hash, ps);
     });
 }
 
@@ -439,7 +446,8 @@ void backward_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const 
     if (M == 0) return;
     // Numeric bin sizes of C's rows (the scan's own output is unused here).
     ws->crpt64.ensure(static_cast<size_t>(M) + 1);
-    launch_scan(ws->row_nnz.p, ws->row_nnz.p, A->rpt, M, ws->block_sums.p, ws->crpt64.p, nullptr, ws->d_stats, s);
+    launch_scan(ws->row_nnz.p, ws->row_nnz.p, A->rpt, M, ws->block_sums.p, ws->crpt64.p, nullptr, ws->d_stats,
+                ws->bounds, s);
     CK(hipGetLastError());
     CK(hipMemcpyAsync(ws->h_stats, ws->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
@@ -456,7 +464,7 @@ void backward_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const 
     }
     const int *rows = nullptr;
     if (!(nonempty == 1 && st.num_size[only] == M) && nonempty > 0) {
-        launch_bin_scatter(ws->row_nnz.p, M, 1, ws->d_stats, ws->bins.p, s);
+        launch_bin_scatter(ws->row_nnz.p, M, 1, ws->d_stats, ws->bins.p, ws->bounds, s);
         CK(hipGetLastError());
         rows = ws->bins.p;
     }
@@ -525,6 +533,57 @@ int ops_workspace_destroy(ops_workspace *ws)
 }
 
 const char *ops_workspace_last_error(const ops_workspace *ws) { return ws ? ws->err.c_str() : "null workspace"; }
+
+int ops_tuning_default(ops_tuning *t)
+{
+    if (!t) return OPS_INVALID;
+    for (int i = 0; i < kNumBins - 1; i++) {
+        t->sym_bound[i] = kSymBound[i];
+        t->num_bound[i] = kNumBound[i];
+    }
+    t->short_hash = kHashLin;
+    return OPS_OK;
+}
+
+int ops_tuning_caps(ops_tuning *c)
+{
+    if (!c) return OPS_INVALID;
+    for (int i = 0; i < kNumBins - 1; i++) {
+        c->sym_bound[i] = kSymCap[i];
+        c->num_bound[i] = kNumCap[i];
+    }
+    c->short_hash = kHashFib;
+    return OPS_OK;
+}
+
+int ops_workspace_set_tuning(ops_workspace *ws, const ops_tuning *t)
+{
+    if (!ws || !t) return OPS_INVALID;
+    if (t->short_hash != kHashLin && t->short_hash != kHashFib) return OPS_INVALID;
+    for (int i = 0; i < kNumBins - 1; i++) {
+        if (t->sym_bound[i] < 0 || t->sym_bound[i] > kSymCap[i]) return OPS_INVALID;
+        if (t->num_bound[i] < 0 || t->num_bound[i] > kNumCap[i]) return OPS_INVALID;
+        if (i > 0 && (t->sym_bound[i] < t->sym_bound[i - 1] || t->num_bound[i] < t->num_bound[i - 1]))
+            return OPS_INVALID;
+    }
+    for (int i = 0; i < kNumBins - 1; i++) {
+        ws->bounds.sym[i] = t->sym_bound[i];
+        ws->bounds.num[i] = t->num_bound[i];
+    }
+    ws->short_hash = t->short_hash;
+    return OPS_OK;
+}
+
+int ops_workspace_get_tuning(const ops_workspace *ws, ops_tuning *t)
+{
+    if (!ws || !t) return OPS_INVALID;
+    for (int i = 0; i < kNumBins - 1; i++) {
+        t->sym_bound[i] = ws->bounds.sym[i];
+        t->num_bound[i] = ws->bounds.num[i];
+    }
+    t->short_hash = ws->short_hash;
+    return OPS_OK;
+}
 
 int ops_csr_free(ops_csr *C)
 {
@@ -618,7 +677,7 @@ int ops_compute_flop_device(ops_workspace *ws, const ops_csr *A, const ops_csr *
         const hipStream_t s = static_cast<hipStream_t>(stream);
         ws->row_nnz.ensure(static_cast<size_t>(A->M) + 1);
         launch_zero_stats(ws->d_stats, s);
-        launch_setup(as_in(A), as_in(B), row_flop, ws->row_nnz.p, ws->d_stats, s);
+        launch_setup(as_in(A), as_in(B), row_flop, ws->row_nnz.p, ws->d_stats, ws->bounds, s);
         CK(hipGetLastError());
         CK(hipMemcpyAsync(ws->h_stats, ws->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
         CK(hipStreamSynchronize(s));

@@ -18,7 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from ._lib import (OPS_F32, OPS_F64, OPS_OK, OpsCsr, OpsCsr64, OpsError, OpsTimings, check, lib)
+from ._lib import (OPS_F32, OPS_F64, OPS_OK, OpsCsr, OpsCsr64, OpsError, OpsTimings, OpsTuning, check, lib)
 from .gen import HostCSR
 
 _HIP = None
@@ -89,10 +89,36 @@ class Workspace:
         self.ptr = C.c_void_p()
         check(lib().ops_workspace_create(C.byref(self.ptr)))
 
+    def set_tuning(self, t) -> None:
+        """Installs bin bounds / hash family (an OpsTuning or its dict form)."""
+        if isinstance(t, dict):
+            t = OpsTuning.from_dict(t)
+        check(lib().ops_workspace_set_tuning(self.ptr, C.byref(t)), self.ptr)
+
+    def get_tuning(self) -> dict:
+        t = OpsTuning()
+        check(lib().ops_workspace_get_tuning(self.ptr, C.byref(t)))
+        return t.as_dict()
+
     def __del__(self):
-        if getattr(self, "ptr", None) and self.ptr.value:
-            lib().ops_workspace_destroy(self.ptr)
-            self.ptr = C.c_void_p()
+        try:
+            if getattr(self, "ptr", None) and self.ptr.value:
+                lib().ops_workspace_destroy(self.ptr)
+                self.ptr = C.c_void_p()
+        except Exception:
+            pass
+
+
+def default_tuning() -> dict:
+    t = OpsTuning()
+    check(lib().ops_tuning_default(C.byref(t)))
+    return t.as_dict()
+
+
+def tuning_caps() -> dict:
+    t = OpsTuning()
+    check(lib().ops_tuning_caps(C.byref(t)))
+    return t.as_dict()
 
 
 _default_ws = None
@@ -142,6 +168,17 @@ class LibCSR:
                     raise RuntimeError(f"hipMemcpy failed: {st}")
         return rpt.astype(np.int64), col, val
 
+    def desc(self) -> OpsCsr:
+        """Input descriptor of this (int32-offset) result, e.g. a packed batch."""
+        if self.wide:
+            raise TypeError("a 64-bit-offset result cannot be an input descriptor")
+        return self.d
+
+    @property
+    def val(self):
+        # dtype carrier for spgemm(); the data stay in library memory
+        return torch.empty(0, dtype=self.dtype)
+
     def free(self):
         if self.d is not None:
             (lib().ops_csr64_free if self.wide else lib().ops_csr_free)(C.byref(self.d))
@@ -186,6 +223,23 @@ def spgemm_safe(A: DeviceCSR, B: DeviceCSR, A_host: HostCSR, B_host: HostCSR, ws
     info = dict(total_flop=flop.value, c_nnz=cnnz.value, free_before=fb.value, free_after=fa.value,
                 total_bytes=tot.value)
     return st, (LibCSR(c, False, torch.float64) if st == OPS_OK else None), info
+
+
+def batch_pack(instances, dtype=torch.float64, stream=None):
+    """Packs device CSR instances block-diagonally (ops_batch_pack).
+
+    Returns (packed LibCSR, row_base, col_base): instance i owns rows
+    [row_base[i], row_base[i+1]) and columns [col_base[i], col_base[i+1])."""
+    n = len(instances)
+    arr = (OpsCsr * max(n, 1))()
+    for i, X in enumerate(instances):
+        arr[i] = X.desc()
+    packed = OpsCsr()
+    rb = np.zeros(n + 1, dtype=np.int64)
+    cb = np.zeros(n + 1, dtype=np.int64)
+    check(lib().ops_batch_pack(n, arr, dtype_code(dtype), C.byref(packed), rb.ctypes.data_as(C.POINTER(C.c_int64)),
+                               cb.ctypes.data_as(C.POINTER(C.c_int64)), _stream_ptr(stream)))
+    return LibCSR(packed, False, dtype), rb, cb
 
 
 class Plan:
@@ -283,4 +337,5 @@ def spgemm_autograd(a_val: torch.Tensor, b_val: torch.Tensor, A: DeviceCSR, B: D
 
 
 __all__ = ["DeviceCSR", "Workspace", "LibCSR", "Plan", "spgemm", "spgemm_safe", "backward", "SpGEMMFunction",
-           "spgemm_autograd", "default_workspace", "OpsTimings", "OpsError", "dtype_code"]
+           "spgemm_autograd", "default_workspace", "OpsTimings", "OpsError", "dtype_code", "batch_pack",
+           "default_tuning", "tuning_caps"]

@@ -91,3 +91,14 @@ def test_invalid_arguments_without_gpu():
 def test_host_flop_matches_oracle():
     A = G.rmat(11, 8, 1)
     assert G.compute_flop_host(A, A) == oracle.compute_flop(A, A)
+
+
+def test_tuning_defaults_within_caps_without_gpu():
+    d, c = _lib.OpsTuning(), _lib.OpsTuning()
+    assert _lib.lib().ops_tuning_default(C.byref(d)) == _lib.OPS_OK
+    assert _lib.lib().ops_tuning_caps(C.byref(c)) == _lib.OPS_OK
+    for i in range(_lib.NUM_BINS - 1):
+        assert 0 < d.sym_bound[i] <= c.sym_bound[i] and 0 < d.num_bound[i] <= c.num_bound[i]
+        if i:
+            assert d.sym_bound[i] >= d.sym_bound[i - 1] and d.num_bound[i] >= d.num_bound[i - 1]
+    assert _lib.lib().ops_workspace_set_tuning(None, C.byref(d)) == _lib.OPS_INVALID

@@ -58,3 +58,43 @@ def test_batch_plan_graph_replays(gpu):
         # eager work between replays
         _ = (dA.val * 2).sum().item()
         oracle.compare(rpt, col.cpu().numpy(), val.cpu().numpy(), ref, 1e-12)
+
+
+def test_batch_pack_matches_generator_packing(gpu):
+    b = G.batch(20, 11, n_lo=64, n_hi=512)
+    insts_a, insts_b = [], []
+    for i in range(b.count):
+        r0, r1 = int(b.row_base[i]), int(b.row_base[i + 1])
+        for X, out in ((b.A, insts_a), (b.B, insts_b)):
+            h = X.row_slice(r0, r1)
+            h = G.HostCSR(h.M, r1 - r0, h.rpt, (h.col - r0).astype(np.int32), h.val)  # local columns
+            out.append(DeviceCSR.from_host(h, torch.float64))
+    from opsparse_amd.spgemm import batch_pack
+    pa, rb, cb = batch_pack(insts_a)
+    assert np.array_equal(rb, b.row_base) and np.array_equal(cb, b.row_base)
+    rpt, col, val = pa.to_host()
+    assert np.array_equal(rpt, b.A.rpt) and np.array_equal(col, b.A.col) and np.array_equal(val, b.A.val)
+    pb, _, _ = batch_pack(insts_b)
+    C = spgemm(pa, pb, wide=True)
+    crpt, ccol, cval = C.to_host()
+    oracle.compare(crpt, ccol, cval, oracle.spgemm(b.A, b.B), 1e-12)
+
+
+def test_every_tuning_candidate_is_exact(gpu):
+    from opsparse_amd import autotune
+    from opsparse_amd.spgemm import Workspace
+    A = G.rmat(12, 10, 13)
+    dA = DeviceCSR.from_host(A, torch.float64)
+    ref = oracle.spgemm(A, A)
+    ws = Workspace()
+    for t in autotune.candidates():
+        ws.set_tuning(t)
+        assert ws.get_tuning() == t
+        rpt, col, val = spgemm(dA, dA, ws=ws).to_host()
+        oracle.compare(rpt, col, val, ref, 1e-12)
+    bad = autotune.candidates()[0]
+    bad["sym_bound"][0] = 10 ** 6  # above the bin-0 table capacity
+    with pytest.raises(Exception):
+        ws.set_tuning(bad)
+    res = autotune.autotune(dA, dA, ws=ws, use_cache=False, reps=1)
+    assert res["tuning"] in autotune.candidates() and len(res["trials"]) == len(autotune.candidates())
