@@ -1,24 +1,35 @@
 """SpGEMM benchmark on one MI355X.
 
-A step is one drop-in call C = A*B (ops_spgemm_f64: setup, binning,
-symbolic, scan, allocation of C, numeric; inputs resident in HBM) followed
-by freeing C, the reference's convention (benchmarks/opsparse_profile.cu:24-39:
-warm-up, then timed iterations with C released between them).
+Default workload: C = A*A for the 27-point stencil on a 100^3 grid (1M rows,
+FP64, seed 1).  A step is one drop-in call (ops_spgemm_f64: setup, binning,
+symbolic, scan, allocation of C, numeric) with inputs resident in HBM; C is
+released outside the timed region, as the reference releases it outside its
+timed call (benchmarks/opsparse_profile.cu:24-39).
 
-Prints ONE JSON line.  value = 2 * products / time in GFLOP/s.  roofline
-= the kernel with the largest HIP-event time in the timed region (events
-recorded on the pool stream the kernel runs on), with its algorithmic bytes
-per launch:
+Other workloads (--workload):
+  uniform  : 4096^2, 8 distinct uniform columns per row, FP64, seed 0
+  rmat20   : R-MAT 2^20 A*A, FP64, seed 2, 64-bit row offsets (nnz(C) > 2^31)
+  batch    : 4096 independent C_i = A_i*B_i packed block-diagonally, one fused
+             autotuned call (and a replayed numeric hipGraph) against the plain
+             call looped over the instances; FP64 line, FP32 in the config
+  backward : R-MAT 2^18 A*B in FP32, forward + masked backward, gradients
+             checked against FP64 direct sums and FP64 finite differences
+
+Prints ONE JSON line.  value = 2 * products / time in GFLOP/s.  roofline =
+the kernel with the largest mean HIP-event time over the timed calls (events
+recorded on the pool stream it runs on), with its algorithmic bytes per
+launch:
   numeric bin  : (4+w)*P + (4+w)*Z + (12+w)*nnzA + 28*rows
   symbolic bin : 4*P + 12*nnzA + 16*rows
 (P products, Z output entries, nnzA A-nonzeros and rows of that bin; w the
-value size).  cpu_baseline = the CPU oracle (OpenMP Gustavson) timed on this
-host's cores over the same matrix.
+value size).  traffic = measured HBM bytes per launch of that kernel when a
+counter summary for it is committed under profiles/.  cpu_baseline = the CPU
+oracle (OpenMP Gustavson) timed on this host's cores over the workload or a
+stated row sample of it; the same oracle output checks the GPU rows.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -29,6 +40,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "SpGEMM GFLOP/s (2×intermediate products/s), % HBM roofline; batched & fwd+bwd"
+KEEP_BYTES = 64 << 30  # keep every timed output alive (freed after timing) below this total
 
 
 def parse():
@@ -36,28 +48,23 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="stencil", choices=["stencil", "uniform"])
+    p.add_argument("--workload", default="stencil", choices=["stencil", "uniform", "rmat20", "batch", "backward"])
     p.add_argument("--cpu-iters", type=int, default=6)
-    p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--no-check", action="store_true")
+    p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle (timing and row checks)")
+    p.add_argument("--no-check", action="store_true", help="skip output checks")
+    p.add_argument("--batch-count", type=int, default=4096)
+    p.add_argument("--fd-entries", type=int, default=1024)
     p.add_argument("--copy-calibration", action="store_true",
                    help="after the run, copy exactly 1 GiB device-to-device (the known-size kernel that "
                         "FETCH_SIZE / WRITE_SIZE counter runs are scaled by)")
     return p.parse_args()
 
 
-def workload(name):
-    import opsparse_amd.gen as G
-    if name == "stencil":
-        A = G.stencil27(100, 1)
-        cfg = {"workload": "C=A*A, A = 27-point stencil on 100^3 (1,000,000 rows), diag 26, off -1+U[-0.01,0.01], "
-                           "FP64, seed 1", "rows": A.M, "nnz_a": A.nnz}
-    else:
-        A = G.uniform_rows(4096, 4096, 8, 0)
-        cfg = {"workload": "C=A*A, A = 4096x4096, 8 uniform distinct columns per row, U[-1,1], FP64, seed 0",
-               "rows": A.M, "nnz_a": A.nnz}
-    return A, cfg
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+
+# ----------------------------------------------------------------------------- timing helpers
 
 def kernel_bytes(kind, b, t, w):
     if kind == "numeric":
@@ -66,113 +73,509 @@ def kernel_bytes(kind, b, t, w):
     return 4 * t.sym_flop[b] + 12 * t.sym_arows_nnz[b] + 16 * t.sym_rows[b]
 
 
+def roofline_of(timings, w, workload_tag):
+    """Dominant kernel over the timed calls, from their per-bin event times."""
+    import numpy as np
+    agg = {}
+    for t in timings:
+        for kind, ms_arr in (("numeric", t.num_ms), ("symbolic", t.sym_ms)):
+            for b in range(8):
+                if ms_arr[b] > 0:
+                    agg.setdefault((kind, b), []).append(ms_arr[b])
+    (kind, b), ms_list = max(agg.items(), key=lambda kv: float(np.mean(kv[1])))
+    ms_k = float(np.mean(ms_list))
+    bytes_k = kernel_bytes(kind, b, timings[-1], w)
+    achieved = bytes_k / (ms_k * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"{kind} bin {b}", "kernel_ms": round(ms_k, 4), "bytes_per_launch": int(bytes_k)}
+    tf = os.path.join(ROOT, "profiles", f"traffic_{kind}_bin{b}_{workload_tag}.txt")
+    if os.path.exists(tf):
+        for ln in open(tf):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                roof["traffic"] = int(float(ln.split(":")[1]))
+    return roof
+
+
+def timed_calls(A, B, ws, steps, warmup, wide):
+    """warmup + steps drop-in calls; returns (timed seconds, per-call timings, last C).
+
+    When all outputs fit in KEEP_BYTES the steps run back to back and the
+    outputs are released after the timed region; otherwise each step is
+    bracketed by synchronisations and its output released between steps,
+    outside the timed spans (the seconds are then their sum).  The output
+    size comes from the first warm-up call (one untimed call is made for it
+    when warmup is 0)."""
+    import torch
+    from opsparse_amd._lib import OpsTimings
+    from opsparse_amd.spgemm import spgemm
+    out_bytes = 0
+    for _ in range(max(warmup, 1)):
+        c = spgemm(A, B, ws=ws, wide=wide)
+        out_bytes = c.nnz * (4 + A.val.element_size()) + (c.M + 1) * (8 if wide else 4)
+        c.free()
+    torch.cuda.synchronize()
+    tims = []
+    if out_bytes * steps <= KEEP_BYTES:
+        outs = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t = OpsTimings()
+            t.kernel_events = 1
+            outs.append(spgemm(A, B, ws=ws, timings=t, wide=wide))
+            tims.append(t)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        for c in outs[:-1]:
+            c.free()
+        return elapsed, tims, outs[-1]
+    elapsed, last = 0.0, None
+    for _ in range(steps):
+        if last is not None:
+            last.free()
+        t = OpsTimings()
+        t.kernel_events = 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        last = spgemm(A, B, ws=ws, timings=t, wide=wide)
+        torch.cuda.synchronize()
+        elapsed += time.perf_counter() - t0
+        tims.append(t)
+    return elapsed, tims, last
+
+
+def cpu_leg(A, B, r0, r1, iters, desc, gpu_c, tol, check):
+    """Times the CPU oracle on rows [r0, r1) of A*B and, unless check is
+    False, compares the GPU result's rows with the oracle's."""
+    import numpy as np
+    import oracle
+    import opsparse_amd.gen as G
+    full = r0 == 0 and r1 == A.M
+
+    def run(with_abs):
+        return oracle.spgemm(A, B, with_abs) if full else oracle.spgemm_rows(A, B, r0, r1, with_abs)
+
+    Ps = G.compute_flop_host(A if full else A.row_slice(r0, r1), B)
+    ref = run(True)  # warm-up run; its |a*b| sums normalise the check
+    times = []
+    for _ in range(iters):
+        s = time.perf_counter()
+        run(False)
+        times.append(time.perf_counter() - s)
+    tc = float(np.mean(times))
+    cpu = {"value": round(2.0 * Ps / tc / 1e9, 3), "unit": "GFLOP/s", "cores": oracle.num_threads(), "kind": "port",
+           "sample": f"{desc} ({Ps} products); OpenMP Gustavson (oracle), 1 warm-up + mean of {iters} runs, "
+                     f"{tc * 1e3:.1f} ms/run, threads={oracle.num_threads()}, host cpus={os.cpu_count()}"}
+    err = None
+    if check:
+        rpt, col, val = gpu_c.rows_to_host(r0, r1)
+        err = oracle.compare(rpt, col, val, ref, tol)
+    return cpu, err
+
+
+def phase_ms(tims):
+    """Mean host-side phase times (ms) and the last call's per-bin kernel event times."""
+    import numpy as np
+    mean = lambda f: round(float(np.mean([getattr(t, f) for t in tims])) * 1e3, 3)
+    return {"setup": mean("setup"), "symbolic_and_scan": mean("prefix"), "allocate_c": mean("allocate"),
+            "numeric": mean("numeric"), "total": mean("total"),
+            "symbolic_bin_ms": [round(x, 3) for x in tims[-1].sym_ms],
+            "numeric_bin_ms": [round(x, 3) for x in tims[-1].num_ms],
+            "symbolic_bin_rows": list(tims[-1].sym_rows), "numeric_bin_rows": list(tims[-1].num_rows)}
+
+
+def line(value, steps, warmup, ms_per_step, dtype, cfg, roof, cpu):
+    return {"metric": METRIC, "value": round(value, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": cfg, "roofline": roof,
+            "cpu_baseline": cpu}
+
+
+# ----------------------------------------------------------------------------- plain products
+
+def run_single(args, name):
+    """stencil / uniform / rmat20: one drop-in call per step."""
+    import numpy as np
+    import torch
+    import opsparse_amd.gen as G
+    from opsparse_amd.spgemm import DeviceCSR, Workspace
+    t_gen = time.time()
+    if name == "stencil":
+        A = G.stencil27(100, 1)
+        desc = ("C=A*A, A = 27-point stencil on 100^3 (1,000,000 rows), diag 26, off -1+U[-0.01,0.01], FP64, "
+                "seed 1")
+    elif name == "uniform":
+        A = G.uniform_rows(4096, 4096, 8, 0)
+        desc = "C=A*A, A = 4096x4096, 8 uniform distinct columns per row, U[-1,1], FP64, seed 0"
+    else:
+        A = G.rmat(20, 16, 2)
+        desc = ("C=A*A, A = R-MAT 2^20, (a,b,c,d)=(.57,.19,.19,.05), 16*2^20 edges merged, no vertex "
+                "permutation, U[-1,1], FP64, seed 2, 64-bit row offsets")
+    P = G.compute_flop_host(A, A)
+    t_gen = time.time() - t_gen
+    wide = name == "rmat20"
+    w = 8
+    dA = DeviceCSR.from_host(A, torch.float64)
+    ws = Workspace()
+    torch.cuda.synchronize()
+    log(f"{name}: rows {A.M} nnz {A.nnz} products {P}")
+    elapsed, tims, c_last = timed_calls(dA, dA, ws, args.steps, args.warmup, wide)
+    nnz_c = c_last.nnz
+    ms_per_step = elapsed / args.steps * 1e3
+    gflops = 2.0 * P / (elapsed / args.steps) / 1e9
+    log(f"{name}: {ms_per_step:.3f} ms/call, {gflops:.1f} GFLOP/s, nnz(C) {nnz_c}")
+    roof = roofline_of(tims, w, name)
+    bytes_alg = (8 + w) * P + (4 + w) * nnz_c + (36 + w) * A.nnz + 32 * A.M
+    cfg = {"workload": desc, "rows": A.M, "nnz_a": A.nnz, "products": int(P), "nnz_c": int(nnz_c),
+           "gen_s": round(t_gen, 2), "ref_convention_ms": round(float(np.mean([t.total for t in tims])) * 1e3, 4),
+           "pipeline_roofline_frac": round(bytes_alg / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+           "bytes_alg": int(bytes_alg), "phases": phase_ms(tims), "parallelism": "single GPU"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        if name == "rmat20":
+            r1 = 1024
+            cpu, err = cpu_leg(A, A, 0, r1, max(1, args.cpu_iters // 3),
+                               "rows 0..1023 of A times A (the highest-degree rows)", c_last, 1e-12, not args.no_check)
+            if err is not None:
+                import oracle
+                worst = err["max_norm"]
+                for s0 in (4096, 262144, 700000, A.M - 4096):  # untimed blocks across the matrix
+                    ref = oracle.spgemm_rows(A, A, s0, s0 + 4096)
+                    rpt, col, val = c_last.rows_to_host(s0, s0 + 4096)
+                    worst = max(worst, oracle.compare(rpt, col, val, ref, 1e-12)["max_norm"])
+                cfg["parity"] = {"structure": "exact on rows 0..1023 and 4 blocks of 4096 rows",
+                                 "max_norm_err": worst}
+        else:
+            cpu, err = cpu_leg(A, A, 0, A.M, args.cpu_iters, "full workload", c_last, 1e-12, not args.no_check)
+            if err is not None:
+                cfg["parity"] = {"structure": "exact", "max_norm_err": err["max_norm"], "max_rel_err": err["max_rel"]}
+    c_last.free()
+    return line(gflops, args.steps, args.warmup, ms_per_step, "f64", cfg, roof, cpu)
+
+
+# ----------------------------------------------------------------------------- batched
+
+def run_batch(args):
+    """Independent C_i = A_i*B_i packed block-diagonally: one fused call
+    (and the replayed numeric hipGraph of its plan) against the plain call
+    looped over the instances (row-slice descriptors of the packed A)."""
+    import numpy as np
+    import torch
+    import opsparse_amd.gen as G
+    from opsparse_amd import autotune
+    from opsparse_amd.spgemm import DeviceCSR, Plan, Workspace, spgemm
+    t_gen = time.time()
+    b = G.batch(args.batch_count, 3, 64, 16384, 4, 32)
+    P = G.compute_flop_host(b.A, b.B)
+    t_gen = time.time() - t_gen
+    n_cpu = min(256, args.batch_count)
+    log(f"batch: {args.batch_count} instances, rows {b.A.M}, nnz(A) {b.A.nnz}, products {P}")
+    ws = Workspace()
+    res = {}
+    for dt, w, tag in ((torch.float64, 8, "f64"), (torch.float32, 4, "f32")):
+        npdt = np.float64 if dt == torch.float64 else np.float32
+        Ah, Bh = b.A.astype(npdt), b.B.astype(npdt)
+        dA = DeviceCSR.from_host(Ah, dt)
+        dB = DeviceCSR.from_host(Bh, dt)
+        tune = autotune.autotune(dA, dB, ws=ws, wide=True, reps=1)
+        log(f"batch {tag}: tuning {tune['tuning']} (cached: {tune['from_cache']})")
+        elapsed, tims, c_last = timed_calls(dA, dB, ws, args.steps, args.warmup, True)
+        t_fused = elapsed / args.steps
+        nnz_c = c_last.nnz
+        log(f"batch {tag}: fused {t_fused * 1e3:.3f} ms, nnz(C) {nnz_c}")
+        cpu, err = None, None
+        if not args.no_cpu_baseline:
+            r1 = int(b.row_base[n_cpu])
+            cpu, err = cpu_leg(Ah, Bh, 0, r1, max(1, args.cpu_iters // 2), f"instances 0..{n_cpu - 1}", c_last,
+                               1e-12 if w == 8 else 1e-5, not args.no_check)
+            if err is not None:
+                import oracle
+                for i in (args.batch_count // 2, args.batch_count - 1):  # untimed instances further in
+                    s0, s1 = int(b.row_base[i]), int(b.row_base[i + 1])
+                    rpt, col, val = c_last.rows_to_host(s0, s1)
+                    e = oracle.compare(rpt, col, val, oracle.spgemm_rows(Ah, Bh, s0, s1), 1e-12 if w == 8 else 1e-5)
+                    err["max_norm"] = max(err["max_norm"], e["max_norm"])
+        c_last.free()
+        # structure reuse: the numeric phase of the fused plan replayed as a hipGraph
+        plan = Plan(dA, dB, ws=ws)
+        col = torch.empty(max(plan.nnz, 1), dtype=torch.int32, device="cuda")
+        val = torch.empty(max(plan.nnz, 1), dtype=dt, device="cuda")
+        plan.numeric(dA, dB, col, val, graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.numeric(dA, dB, col, val, graph=True)
+        torch.cuda.synchronize()
+        t_graph = (time.perf_counter() - t0) / args.steps
+        del col, val, plan
+        torch.cuda.empty_cache()
+        # the plain call looped over the instances
+        for i in range(2):  # warm-up
+            spgemm(DeviceCSR(int(b.row_base[i + 1] - b.row_base[i]), dA.N,
+                             dA.rpt[int(b.row_base[i]):int(b.row_base[i + 1]) + 1], dA.col, dA.val), dB, ws=ws).free()
+        t_loop = 0.0
+        for i in range(args.batch_count):
+            r0, r1 = int(b.row_base[i]), int(b.row_base[i + 1])
+            Ai = DeviceCSR(r1 - r0, dA.N, dA.rpt[r0:r1 + 1], dA.col, dA.val)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ci = spgemm(Ai, dB, ws=ws)
+            torch.cuda.synchronize()
+            t_loop += time.perf_counter() - t0
+            ci.free()
+        log(f"batch {tag}: graph numeric {t_graph * 1e3:.3f} ms, looped plain call {t_loop * 1e3:.1f} ms")
+        res[tag] = {"t_fused": t_fused, "t_graph": t_graph, "t_loop": t_loop, "tims": tims, "nnz_c": nnz_c,
+                    "phases": phase_ms(tims),
+                    "tuning": tune["tuning"], "cpu": cpu, "err": err}
+        del dA, dB
+        torch.cuda.empty_cache()
+    r, r32 = res["f64"], res["f32"]
+    roof = roofline_of(r["tims"], 8, "batch")
+    cfg = {"workload": f"{args.batch_count} independent C_i=A_i*B_i, n_i log-uniform in [64,16384], nnz/row uniform "
+                       f"in [4,32] per matrix, uniform distinct columns, U[-1,1], seed 3; packed block-diagonally "
+                       f"and run as one fused, autotuned call (FP64 line; FP32 under fp32_*)",
+           "instances": args.batch_count, "rows": int(b.A.M), "nnz_a": int(b.A.nnz), "nnz_b": int(b.B.nnz),
+           "products": int(P), "nnz_c": int(r["nnz_c"]), "gen_s": round(t_gen, 1),
+           "fused_ms": round(r["t_fused"] * 1e3, 3), "graph_numeric_replay_ms": round(r["t_graph"] * 1e3, 3),
+           "looped_plain_call_ms": round(r["t_loop"] * 1e3, 3),
+           "speedup_fused_vs_loop": round(r["t_loop"] / r["t_fused"], 2),
+           "speedup_graph_replay_vs_loop": round(r["t_loop"] / r["t_graph"], 2),
+           "fp32_fused_ms": round(r32["t_fused"] * 1e3, 3),
+           "fp32_gflops": round(2.0 * P / r32["t_fused"] / 1e9, 3),
+           "fp32_graph_numeric_replay_ms": round(r32["t_graph"] * 1e3, 3),
+           "fp32_looped_plain_call_ms": round(r32["t_loop"] * 1e3, 3),
+           "fp32_speedup_fused_vs_loop": round(r32["t_loop"] / r32["t_fused"], 2),
+           "tuning_f64": r["tuning"], "tuning_f32": r32["tuning"], "phases_f64": r["phases"],
+           "phases_f32": r32["phases"], "parallelism": "single GPU"}
+    if r["err"] is not None:
+        cfg["parity"] = {"structure": f"exact on instances 0..{n_cpu - 1}, middle and last",
+                         "max_norm_err_f64": r["err"]["max_norm"], "max_norm_err_f32": r32["err"]["max_norm"]}
+    return line(2.0 * P / r["t_fused"] / 1e9, args.steps, args.warmup, r["t_fused"] * 1e3, "f64", cfg, roof, r["cpu"])
+
+
+# ----------------------------------------------------------------------------- forward + backward
+
+def _gather_rows(rpt, rows):
+    """Indices of the entries of `rows` (in that order) of a CSR with row offsets rpt."""
+    import numpy as np
+    starts, ends = rpt[rows], rpt[rows + 1]
+    lens = ends - starts
+    tot = int(lens.sum())
+    first = np.cumsum(lens) - lens
+    return np.repeat(starts - first, lens) + np.arange(tot, dtype=np.int64)
+
+
+def gradient_checks(A, B, dA64, dB64, crpt_h, ccol_h, dc_h, ga32, gb32, ga64, gb64, n_entries, ws):
+    """FP64 checks of sampled gradient entries: direct sums on the host
+    (dA[i,k] = sum_j dC[i,j] B[k,j], dB[k,j] = sum_i A[i,k] dC[i,j]) and
+    central differences of L = <dC, A*B> through the GPU forward, restricted
+    to the rows of C an entry feeds.  Errors are normalised by the sum of
+    |terms| of the entry's gradient."""
+    import numpy as np
+    import torch
+    from opsparse_amd.gen import HostCSR
+    from opsparse_amd.spgemm import DeviceCSR, spgemm
+    rng = np.random.default_rng(4)
+    a_rows = np.repeat(np.arange(A.M, dtype=np.int64), np.diff(A.rpt.astype(np.int64)))
+    b_rows = np.repeat(np.arange(B.M, dtype=np.int64), np.diff(B.rpt.astype(np.int64)))
+    order = np.argsort(A.col, kind="stable")  # A by column: rows feeding each row of B
+    cstart = np.zeros(A.N + 1, dtype=np.int64)
+    np.cumsum(np.bincount(A.col, minlength=A.N), out=cstart[1:])
+    ga32_h, gb32_h = ga32.double().cpu().numpy(), gb32.double().cpu().numpy()
+    ga64_h, gb64_h = ga64.cpu().numpy(), gb64.cpu().numpy()
+    # L is linear in each entry, so the central difference is exact up to
+    # rounding for any step; a large step keeps the rounding of the other
+    # terms of the affected rows (which do not scale with it) negligible.
+    h = 1024.0
+
+    def sub_csr(rows, k=None, j_local=None, delta=0.0):
+        idx = _gather_rows(A.rpt.astype(np.int64), rows)
+        rpt = np.zeros(len(rows) + 1, dtype=np.int32)
+        np.cumsum(np.diff(A.rpt.astype(np.int64))[rows], out=rpt[1:])
+        val = A.val[idx].astype(np.float64)
+        if j_local is not None:
+            val[j_local] += delta
+        return HostCSR(len(rows), A.N, rpt, A.col[idx].astype(np.int32), val)
+
+    def c_rows(Ahost, Bdev):
+        dAh = DeviceCSR.from_host(Ahost, torch.float64)
+        c = spgemm(dAh, Bdev, ws=ws, wide=True)
+        rpt, col, val = c.to_host()
+        c.free()
+        return rpt, col, val
+
+    out = {"fd64": 0.0, "ref64": 0.0, "ref32": 0.0, "zero_entries": 0, "zero_max_abs_fd": 0.0,
+           "zero_max_abs_grad": 0.0}
+
+    def record(fd, ref, scale, g64, g32):
+        if scale == 0.0:  # no terms: the gradient is structurally zero
+            out["zero_entries"] += 1
+            out["zero_max_abs_fd"] = max(out["zero_max_abs_fd"], abs(fd))
+            out["zero_max_abs_grad"] = max(out["zero_max_abs_grad"], abs(g64), abs(g32))
+            return
+        out["fd64"] = max(out["fd64"], abs(fd - g64) / scale)
+        out["ref64"] = max(out["ref64"], abs(ref - g64) / scale)
+        out["ref32"] = max(out["ref32"], abs(ref - g32) / scale)
+    n_half = n_entries // 2
+    for cnt, e in enumerate(rng.choice(A.nnz, n_half, replace=False)):
+        if cnt % 128 == 0:
+            log(f"gradient check: dA entry {cnt}/{n_half}")
+        i, k = int(a_rows[e]), int(A.col[e])
+        crow = ccol_h[crpt_h[i]:crpt_h[i + 1]]
+        bj = B.col[B.rpt[k]:B.rpt[k + 1]]
+        bv = B.val[B.rpt[k]:B.rpt[k + 1]].astype(np.float64)
+        pos = np.searchsorted(crow, bj)
+        assert np.array_equal(crow[pos], bj), "pattern(C) misses a product of a row of A"
+        dcr = dc_h[crpt_h[i] + pos]
+        ref, scale = float(np.dot(dcr, bv)), float(np.dot(np.abs(dcr), np.abs(bv)))
+        # FD: row i with A[i,k] +- h, both in one call
+        local = e - int(A.rpt[i])
+        Hp = sub_csr(np.array([i]), j_local=local, delta=h)
+        Hm = sub_csr(np.array([i]), j_local=local, delta=-h)
+        step = float(Hp.val[local]) - float(Hm.val[local])
+        two = HostCSR(2, A.N, np.concatenate([Hp.rpt, Hp.rpt[1:] + Hp.rpt[-1]]).astype(np.int32),
+                      np.concatenate([Hp.col, Hm.col]), np.concatenate([Hp.val, Hm.val]))
+        rpt, col, val = c_rows(two, dB64)
+        n = int(rpt[1])
+        assert np.array_equal(col[:n], crow) and np.array_equal(col[n:], crow)
+        fd = float(np.dot(dc_h[crpt_h[i]:crpt_h[i + 1]], val[:n] - val[n:])) / step
+        record(fd, ref, scale, ga64_h[e], ga32_h[e])
+    for cnt, e in enumerate(rng.choice(B.nnz, n_entries - n_half, replace=False)):
+        if cnt % 128 == 0:
+            log(f"gradient check: dB entry {cnt}/{n_entries - n_half}")
+        k, j = int(b_rows[e]), int(B.col[e])
+        feed = order[cstart[k]:cstart[k + 1]]
+        rows = a_rows[feed]
+        av = A.val[feed].astype(np.float64)
+        dcv = np.empty(len(rows))
+        for t, r in enumerate(rows):
+            crow = ccol_h[crpt_h[r]:crpt_h[r + 1]]
+            p = int(np.searchsorted(crow, j))
+            assert p < crow.size and crow[p] == j
+            dcv[t] = dc_h[crpt_h[r] + p]
+        ref, scale = float(np.dot(av, dcv)), float(np.dot(np.abs(av), np.abs(dcv)))
+        fd = 0.0
+        if len(rows):
+            sub = sub_csr(rows)
+            res, pert = [], []
+            for s in (h, -h):
+                bval = dB64.val.clone()
+                bval[int(e)] += s
+                pert.append(float(bval[int(e)]))
+                res.append(c_rows(sub, dB64.with_values(bval)))
+            (rp, cp, vp), (_, cm, vm) = res
+            idx = _gather_rows(crpt_h, rows)
+            assert np.array_equal(cp, ccol_h[idx]) and np.array_equal(cm, cp)
+            fd = float(np.dot(dc_h[idx], vp - vm)) / (pert[0] - pert[1])
+        record(fd, ref, scale, gb64_h[e], gb32_h[e])
+    return out
+
+
+def run_backward(args):
+    """R-MAT 2^18 A*B in FP32: forward (drop-in call) + masked backward."""
+    import numpy as np
+    import torch
+    import opsparse_amd.gen as G
+    from opsparse_amd.spgemm import DeviceCSR, Plan, Workspace, backward
+    t_gen = time.time()
+    A = G.rmat(18, 12, 4, normal=True)
+    B = G.rmat(18, 12, 1004, normal=True)
+    P = G.compute_flop_host(A, B)
+    t_gen = time.time() - t_gen
+    log(f"backward: nnz(A) {A.nnz} nnz(B) {B.nnz} products {P}")
+    ws = Workspace()
+    A32, B32 = A.astype(np.float32), B.astype(np.float32)
+    dA = DeviceCSR.from_host(A32, torch.float32)
+    dB = DeviceCSR.from_host(B32, torch.float32)
+    elapsed, tims, c_last = timed_calls(dA, dB, ws, args.steps, args.warmup, True)
+    t_fwd = elapsed / args.steps
+    plan = Plan(dA, dB, ws=ws)
+    rpt = plan.rowptr64()
+    col, _ = plan.numeric(dA, dB)
+    for _ in range(args.warmup):
+        plan.numeric(dA, dB, col)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.numeric(dA, dB, col)
+    torch.cuda.synchronize()
+    t_num = (time.perf_counter() - t0) / args.steps
+    g = torch.Generator(device="cuda").manual_seed(4)
+    dc = torch.randn(plan.nnz, dtype=torch.float32, device="cuda", generator=g)
+    for _ in range(args.warmup):
+        backward(dA, dB, rpt, col, dc, ws=ws)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ga, gb = backward(dA, dB, rpt, col, dc, ws=ws)
+    torch.cuda.synchronize()
+    t_bwd = (time.perf_counter() - t0) / args.steps
+    log(f"backward: forward call {t_fwd * 1e3:.3f} ms, plan numeric {t_num * 1e3:.3f} ms, backward "
+        f"{t_bwd * 1e3:.3f} ms")
+    cfg = {"workload": "C=A*B, A and B independent R-MAT 2^18 (.57,.19,.19,.05), 12*2^18 edges merged, N(0,1) "
+                       "values in FP32, seeds 4 and 1004; dC ~ N(0,1) on pattern(C); forward = drop-in call, "
+                       "backward = masked dA on pattern(A), dB on pattern(B)",
+           "rows": A.M, "nnz_a": A.nnz, "nnz_b": B.nnz, "products": int(P), "nnz_c": int(plan.nnz),
+           "gen_s": round(t_gen, 1), "forward_call_ms": round(t_fwd * 1e3, 3),
+           "plan_numeric_ms": round(t_num * 1e3, 3), "backward_ms": round(t_bwd * 1e3, 3),
+           "fwd_plus_bwd_over_fwd": round((t_fwd + t_bwd) / t_fwd, 3),
+           "numeric_plus_bwd_over_numeric": round((t_num + t_bwd) / t_num, 3), "phases": phase_ms(tims),
+           "parallelism": "single GPU"}
+    if not args.no_check:
+        crpt_h, ccol_h = rpt.cpu().numpy(), col.cpu().numpy()
+        dA64, dB64 = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+        dc64 = dc.double()
+        ga64, gb64 = backward(dA64, dB64, rpt, col, dc64, ws=ws)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        chk = gradient_checks(A, B, dA64, dB64, crpt_h, ccol_h, dc64.cpu().numpy(), ga, gb, ga64, gb64,
+                              args.fd_entries, ws)
+        cfg["gradient_check"] = {
+            "entries": args.fd_entries, "normalisation": "sum of |terms| of the entry's gradient",
+            "fp64_vs_central_difference_max_err": chk["fd64"], "fp64_vs_fp64_direct_sum_max_err": chk["ref64"],
+            "fp32_vs_fp64_direct_sum_max_err": chk["ref32"], "structurally_zero_entries": chk["zero_entries"],
+            "zero_entries_max_abs_fd": chk["zero_max_abs_fd"], "zero_entries_max_abs_grad": chk["zero_max_abs_grad"],
+            "seconds": round(time.time() - t0, 1)}
+        log(f"backward: gradient checks {cfg['gradient_check']}")
+    roof = roofline_of(tims, 4, "backward")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu, err = cpu_leg(A32, B32, 0, 4096, max(1, args.cpu_iters // 3),
+                           "forward, rows 0..4095 of A times B (the highest-degree rows)", c_last, 1e-5,
+                           not args.no_check)
+        if err is not None:
+            cfg["parity"] = {"structure": "exact on rows 0..4095", "max_norm_err": err["max_norm"]}
+            import oracle
+            # FP64 CPU reference gradient dA on rows 0..4095 (those rows of dA depend on those rows of C only)
+            s0, s1 = int(crpt_h[0]), int(crpt_h[4096])
+            Ar = A.row_slice(0, 4096)
+            ra, _ = oracle.backward(Ar, B, crpt_h[:4097] - s0, ccol_h[s0:s1], dc64[s0:s1].cpu().numpy())
+            Aabs = G.HostCSR(Ar.M, Ar.N, Ar.rpt, Ar.col, np.abs(Ar.val))
+            Babs = G.HostCSR(B.M, B.N, B.rpt, B.col, np.abs(B.val))
+            sa, _ = oracle.backward(Aabs, Babs, crpt_h[:4097] - s0, ccol_h[s0:s1], np.abs(dc64[s0:s1].cpu().numpy()))
+            a1 = int(A.rpt[4096])
+            sc = np.maximum(sa, 1e-300)
+            cfg["gradient_check"]["oracle_dA_rows_0_4095"] = {
+                "fp64_max_err": float(np.max(np.abs(ga64[:a1].cpu().numpy() - ra) / sc, initial=0.0)),
+                "fp32_max_err": float(np.max(np.abs(ga[:a1].double().cpu().numpy() - ra) / sc, initial=0.0))}
+    c_last.free()
+    return line(2.0 * P / t_fwd / 1e9, args.steps, args.warmup, t_fwd * 1e3, "f32", cfg, roof, cpu)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is "
               f"design-only in this build", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
-
-    import opsparse_amd.gen as G
-    from opsparse_amd._lib import OpsTimings, lib
-    from opsparse_amd.spgemm import DeviceCSR, Workspace, spgemm
-
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    t_gen = time.time()
-    A, cfg = workload(args.workload)
-    P = G.compute_flop_host(A, A)
-    t_gen = time.time() - t_gen
-    w = 8
-    dA = DeviceCSR.from_host(A, torch.float64)
-    ws = Workspace()
-    torch.cuda.synchronize()
-
-    for _ in range(args.warmup):
-        spgemm(dA, dA, ws=ws).free()
-    torch.cuda.synchronize()
-
-    agg = {}
-    per_call = []
-    nnz_c = 0
-    outs = []
-    timings = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        t = OpsTimings()
-        t.kernel_events = 1
-        # Each call allocates and computes its own C; releasing C happens after
-        # the timed region, as the reference releases it outside the timed call.
-        outs.append(spgemm(dA, dA, ws=ws, timings=t))
-        timings.append(t)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    for t in timings:
-        per_call.append(t.total)
-        for kind, ms_arr in (("numeric", t.num_ms), ("symbolic", t.sym_ms)):
-            for b in range(8):
-                if ms_arr[b] > 0:
-                    agg.setdefault((kind, b), []).append(ms_arr[b])
-    c_last, t_last = outs[-1], timings[-1]
-    nnz_c = c_last.nnz
-    for c in outs[:-1]:
-        c.free()
-    ms_per_step = elapsed / args.steps * 1e3
-    gflops = 2.0 * P / (elapsed / args.steps) / 1e9
-
-    # dominant kernel and its roofline
-    (kind, b), ms_list = max(agg.items(), key=lambda kv: float(np.mean(kv[1])))
-    ms_k = float(np.mean(ms_list))
-    bytes_k = kernel_bytes(kind, b, t_last, w)
-    achieved = bytes_k / (ms_k * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"{kind} bin {b}", "kernel_ms": round(ms_k, 4), "bytes_per_launch": int(bytes_k)}
-    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{kind}_bin{b}_{args.workload}.txt")
-    if os.path.exists(traffic_file):
-        for line in open(traffic_file):
-            if line.startswith("traffic_bytes_per_launch:"):
-                roofline["traffic"] = int(float(line.split(":")[1]))
-
-    # whole-pipeline algorithmic bytes (setup + symbolic + numeric)
-    bytes_alg = (8 + w) * P + (4 + w) * nnz_c + (36 + w) * A.nnz + 32 * A.M
-    pipe_frac = bytes_alg / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS
-
-    check = None
-    if not args.no_check:
-        import oracle
-        rpt, col, val = c_last.to_host()
-        ref = oracle.spgemm(A, A)
-        check = oracle.compare(rpt, col, val, ref, 1e-12)
-    c_last.free()
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        import oracle
-        times = []
-        oracle.spgemm(A, A, with_abs=False)  # warm-up
-        for _ in range(args.cpu_iters):
-            s = time.perf_counter()
-            oracle.spgemm(A, A, with_abs=False)
-            times.append(time.perf_counter() - s)
-        tc = float(np.mean(times))
-        cpu = {"value": round(2.0 * P / tc / 1e9, 3), "unit": "GFLOP/s", "cores": oracle.num_threads(),
-               "kind": "port",
-               "sample": f"full workload, OpenMP Gustavson (oracle), 1 warm-up + mean of {args.cpu_iters} runs, "
-                         f"{tc * 1e3:.1f} ms/run, threads={oracle.num_threads()}, host cpus={os.cpu_count()}"}
-
-    cfg.update({"products": int(P), "nnz_c": int(nnz_c), "gen_s": round(t_gen, 2),
-                "ref_convention_ms": round(float(np.mean(per_call)) * 1e3, 4),
-                "pipeline_roofline_frac": round(pipe_frac, 4), "bytes_alg": int(bytes_alg),
-                "parallelism": "single GPU"})
-    if check is not None:
-        cfg["parity"] = {"structure": "exact", "max_norm_err": check["max_norm"], "max_rel_err": check["max_rel"]}
-    out = {"metric": METRIC, "value": round(gflops, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": cfg,
-           "roofline": roofline, "cpu_baseline": cpu}
+    if args.workload in ("stencil", "uniform", "rmat20"):
+        out = run_single(args, args.workload)
+    elif args.workload == "batch":
+        out = run_batch(args)
+    else:
+        out = run_backward(args)
     print(json.dumps(out), flush=True)
     if args.copy_calibration:
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")

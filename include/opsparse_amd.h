@@ -106,8 +106,9 @@ const char *ops_workspace_last_error(const ops_workspace *ws);
 /*
  * C = A * B.  Replaces opsparse::opsparse_spgemm / opsparse_spgemm_reg
  * (include/opsparse/core_algo/spgemm/spgemm.hpp:12-17; src/opsparse/core_algo/spgemm/spgemm.cu:16-126).
- * The library allocates C->rpt, C->col and C->val with hipMalloc (free them
- * with ops_csr_free, the counterpart of CSR::release, csr.cu:40-56) and sets
+ * The library allocates C->rpt, C->col and C->val (from its output cache,
+ * below; release them with ops_csr_free, the counterpart of CSR::release,
+ * csr.cu:40-56) and sets
  * C->M, C->N, C->nnz.  Returns when C is complete on `stream`.  A and B must
  * have sorted, duplicate-free rows; A->N must equal B->M.
  */
@@ -119,9 +120,32 @@ int ops_spgemm_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_cs
 int ops_spgemm64_f64(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr64 *C, void *stream, ops_timings *t);
 int ops_spgemm64_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr64 *C, void *stream, ops_timings *t);
 
-/* Frees library-allocated C arrays and zeroes the descriptor (CSR::drelease, csr.cu:40-48). */
+/* Releases library-allocated C arrays and zeroes the descriptor (CSR::drelease, csr.cu:40-48). */
 int ops_csr_free(ops_csr *C);
 int ops_csr64_free(ops_csr64 *C);
+
+/*
+ * Output memory (no counterpart in the reference, which calls cudaMalloc for
+ * C on every product: spgemm.cu:96-114).  C's arrays and packed batches come
+ * from a per-device cache of blocks released earlier through ops_free,
+ * ops_csr_free or ops_csr64_free; only a miss calls hipMalloc, whose fresh
+ * VRAM the driver clears at a cost comparable to a large product.
+ * Releasing synchronises the device, as hipFree does.  Library-allocated
+ * arrays must be released through these calls, not hipFree; ops_free passes
+ * any other pointer to hipFree.  The cache keeps at most `bytes` per device
+ * (bytes < 0, the default: half of the device's memory; 0: no caching).
+ */
+typedef struct {
+    int64_t cached_bytes, cached_blocks;  /* held by the cache */
+    int64_t live_bytes, live_blocks;      /* handed out, not yet released */
+    int64_t hits, misses;                 /* allocations served from the cache / by hipMalloc */
+    int64_t limit;                        /* -1: default cap */
+} ops_mem_stats;
+
+int ops_free(void *p);
+int ops_mem_cache_set_limit(int64_t bytes);
+int ops_mem_cache_release(void);
+int ops_mem_cache_stats(ops_mem_stats *out);
 
 /*
  * Safe variant: never throws, classifies failures.  Replaces

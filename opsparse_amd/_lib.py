@@ -79,6 +79,14 @@ class OpsHostCsr(C.Structure):
                 ("col", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double))]
 
 
+class OpsMemStats(C.Structure):
+    _fields_ = [("cached_bytes", C.c_int64), ("cached_blocks", C.c_int64), ("live_bytes", C.c_int64),
+                ("live_blocks", C.c_int64), ("hits", C.c_int64), ("misses", C.c_int64), ("limit", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 # name -> (restype, argtypes); every function declared in include/opsparse_amd.h
 _vp = C.c_void_p
 _P = C.POINTER
@@ -92,6 +100,10 @@ SIGNATURES = {
     "ops_spgemm64_f32": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _P(OpsCsr64), _vp, _P(OpsTimings)]),
     "ops_csr_free": (C.c_int, [_P(OpsCsr)]),
     "ops_csr64_free": (C.c_int, [_P(OpsCsr64)]),
+    "ops_free": (C.c_int, [_vp]),
+    "ops_mem_cache_set_limit": (C.c_int, [C.c_int64]),
+    "ops_mem_cache_release": (C.c_int, []),
+    "ops_mem_cache_stats": (C.c_int, [_P(OpsMemStats)]),
     "ops_spgemm_safe_f64": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _vp, _vp, _vp, _P(OpsCsr), _vp, _P(OpsTimings),
                                       _P(C.c_int64), _P(C.c_int64), _P(C.c_size_t), _P(C.c_size_t),
                                       _P(C.c_size_t)]),

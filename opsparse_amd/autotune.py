@@ -32,7 +32,7 @@ def _cache_path() -> str:
 
 def signature(A, B) -> str:
     name = torch.cuda.get_device_properties(0).name if torch.cuda.is_available() else "cpu"
-    return f"{name}|{A.M}x{A.N}x{B.N}|nnzA={A.nnz}|nnzB={B.nnz}"
+    return f"{name}|{A.M}x{A.N}x{B.N}|nnzA={A.nnz}|nnzB={B.nnz}|{str(A.val.dtype).replace('torch.', '')}"
 
 
 def candidates() -> list[dict]:

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "opsparse_amd.h"
+#include "outmem.hpp"
 
 namespace {
 
@@ -84,9 +85,13 @@ extern "C" int ops_batch_pack(int32_t count, const ops_csr *instances, ops_dtype
         return st;
     };
     hipError_t e;
-    if ((e = hipMalloc(&packed->rpt, sizeof(int) * (static_cast<size_t>(rows) + 1))) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&packed->col, sizeof(int) * static_cast<size_t>(nnz > 0 ? nnz : 1))) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&packed->val, vsz * static_cast<size_t>(nnz > 0 ? nnz : 1))) != hipSuccess) return fail(e);
+    if ((e = opsamd::out_malloc(reinterpret_cast<void **>(&packed->rpt), sizeof(int) * (static_cast<size_t>(rows) + 1))) !=
+        hipSuccess)
+        return fail(e);
+    if ((e = opsamd::out_malloc(reinterpret_cast<void **>(&packed->col), sizeof(int) * static_cast<size_t>(nnz > 0 ? nnz : 1))) !=
+        hipSuccess)
+        return fail(e);
+    if ((e = opsamd::out_malloc(&packed->val, vsz * static_cast<size_t>(nnz > 0 ? nnz : 1))) != hipSuccess) return fail(e);
     packed->M = static_cast<int32_t>(rows);
     packed->N = static_cast<int32_t>(cols);
     packed->nnz = static_cast<int32_t>(nnz);

@@ -18,6 +18,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 #include "launch.hpp"
 
@@ -131,8 +132,22 @@ __device__ __forceinline__ int unit_max(int v)
 
 // Product consumers have two steps so the walkers below can issue the
 // global reads of two products before either LDS update:
-//   Item load(int b_index) const;  // global reads of one B entry
-//   void apply(const Item &, V a);  // LDS / atomic update with A's value
+//   Item load(int b_index) const;          // global reads of one B entry
+//   void apply(const Item &, V a, int q);  // update with A's value; q is the
+//                                          // product's A-nonzero within the
+//                                          // chunk (0..U-1)
+// A consumer may also define
+//   void chunk_end(int e, bool valid, bool shared);
+// which the walk calls once per chunk after all of the chunk's products
+// (e: this lane's A-nonzero index, valid: e is inside the row segment,
+// shared: other units walked slices of the same chunk).
+
+template <typename Op, typename = void>
+struct has_chunk_end : std::false_type {
+};
+template <typename Op>
+struct has_chunk_end<Op, std::void_t<decltype(&Op::chunk_end)>> : std::true_type {
+};
 
 // Grouped walk of one loaded chunk: GS lanes per A-nonzero, each group
 // strides over its B row two entries at a time.  No per-product cross-lane
@@ -162,8 +177,8 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
             const typename Op::Item i0 = op.load(qb + t);
             typename Op::Item i1{};
             if (two) i1 = op.load(qb + t + GS);
-            op.apply(i0, qa);
-            if (two) op.apply(i1, qa);
+            op.apply(i0, qa, q);
+            if (two) op.apply(i1, qa, q);
         }
     }
 }
@@ -219,8 +234,8 @@ __device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op, int pa
         typename Op::Item i0{}, i1{};
         if (ok0) i0 = op.load(b0 + (p0 - v0));
         if (ok1) i1 = op.load(b1 + (p1 - v1));
-        if (ok0) op.apply(i0, a0);
-        if (ok1) op.apply(i1, a1);
+        if (ok0) op.apply(i0, a0, q0);
+        if (ok1) op.apply(i1, a1, q1);
     }
 }
 
@@ -284,8 +299,61 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
         } else {
             chunk_flat<U, WITH_A, V>(bs, len, av, op, part, nparts);
         }
+        if constexpr (has_chunk_end<Op>::value) op.chunk_end(e, e < a_end, shared);
     }
 }
+
+template <typename V>
+struct DenseWin {
+    static constexpr int W = sizeof(V) == 8 ? 16384 : 32768;  // columns per window: 128 KB of values
+    static constexpr int WORDS = W / 32;
+    static constexpr size_t lds = sizeof(V) * W + sizeof(unsigned) * WORDS;
+};
+
+// Cursor-carrying B range for window [w0, wend): lo is the first entry of
+// the B row not consumed by earlier windows; hi the first entry >= wend,
+// found by galloping from lo.  Tracks the smallest column beyond the window
+// so empty windows can be skipped.
+struct CursorRange {
+    const int *__restrict__ brpt;
+    const int *__restrict__ bcol;
+    const int *__restrict__ cur_in;  // position after the previous window (unused on the first)
+    int *__restrict__ cur_out;       // position after this window
+    int wend;
+    bool first;
+    bool last;  // wend covers all columns
+    int *next_col;
+    __device__ __forceinline__ void operator()(int e, int k, int &bs, int &len) const
+    {
+        const int t = brpt[k + 1];
+        const int lo = first ? brpt[k] : cur_in[e];
+        int hi;
+        if (last) {
+            hi = t;
+        } else if (lo >= t || bcol[lo] >= wend) {
+            hi = lo;
+        } else {
+            int prev = lo, step = 1;
+            while (prev + step < t && bcol[prev + step] < wend) {
+                prev += step;
+                step <<= 1;
+            }
+            int l = prev + 1, h = min(prev + step, t);
+            while (l < h) {
+                const int m = (l + h) >> 1;
+                if (bcol[m] < wend)
+                    l = m + 1;
+                else
+                    h = m;
+            }
+            hi = l;
+        }
+        cur_out[e] = hi;
+        if (hi < t) *next_col = min(*next_col, bcol[hi]);
+        bs = lo;
+        len = hi - lo;
+    }
+};
 
 // Home slot of a column key, two families:
 //  * LIN: the reference's multiplicative hash col * 107 mod T

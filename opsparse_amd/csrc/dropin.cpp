@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "opsparse/core_algo/spgemm/spgemm.hpp"
+#include "outmem.hpp"
 #include "opsparse/io/matrix_market.hpp"
 #include "opsparse/numeric/compute_flop.hpp"
 #include "opsparse/utils/tsc_timer.hpp"
@@ -39,9 +40,11 @@ void CSR::hrelease()
 
 void CSR::drelease()
 {
-    if (d_rpt) OPSPARSE_CHECK_HIP(hipFree(d_rpt));
-    if (d_col) OPSPARSE_CHECK_HIP(hipFree(d_col));
-    if (d_val) OPSPARSE_CHECK_HIP(hipFree(d_val));
+    // ops_free: arrays of a library-computed C go back to the output cache,
+    // arrays from H2D (plain hipMalloc) to hipFree.
+    if (d_rpt) OPSPARSE_CHECK_HIP(opsamd::out_free(d_rpt));
+    if (d_col) OPSPARSE_CHECK_HIP(opsamd::out_free(d_col));
+    if (d_val) OPSPARSE_CHECK_HIP(opsamd::out_free(d_val));
     d_rpt = nullptr;
     d_col = nullptr;
     d_val = nullptr;
@@ -57,9 +60,9 @@ CSR::~CSR()
 {
     hrelease();
     // Destructors must not throw; ignore device errors here.
-    if (d_rpt) (void)hipFree(d_rpt);
-    if (d_col) (void)hipFree(d_col);
-    if (d_val) (void)hipFree(d_val);
+    if (d_rpt) (void)opsamd::out_free(d_rpt);
+    if (d_col) (void)opsamd::out_free(d_col);
+    if (d_val) (void)opsamd::out_free(d_val);
 }
 
 void CSR::H2D()

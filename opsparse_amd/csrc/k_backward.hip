@@ -1,13 +1,20 @@
 // Masked backward of C = A*B at fixed sparsity (no reference counterpart):
 //   dA(i,k) = sum_j dC(i,j) * B(k,j)   for (i,k) in pattern(A)
 //   dB(k,j) = sum_i A(i,k) * dC(i,j)   for (k,j) in pattern(B)
-// Every forward product (i,k,j) contributes once to each.  Row i of C (its
-// sorted columns and dC values) is loaded into the same LDS open-addressing
-// table the forward numeric kernels use, keyed by column, and the products
-// of row i are walked with the same flattened per-unit iterator; dA is
-// reduced per A-nonzero in LDS, dB is scattered with global atomic adds.
-// Rows of C above 4096 entries look dC up by binary search in the sorted
-// C row instead of an LDS table.
+// Every forward product (i,k,j) contributes once to each.  Rows are binned
+// by the length of row i of C, and the products of row i are walked with
+// the forward kernels' product walk (for_each_product); dC(i,j) is looked
+// up per product:
+//  * rows up to 4096 entries: row i of C (its columns and dC values) is
+//    loaded into the same LDS open-addressing table the forward numeric
+//    kernels use, keyed by column;
+//  * longer rows: the row is split into dense column windows; each window's
+//    dC values are staged in LDS indexed by column, and B rows are consumed
+//    window by window through per-A-nonzero cursors (as in the forward's
+//    long-row kernel).  Windows without entries in row i of C hold no
+//    products and are skipped by reading the next column of the C row.
+// dA is reduced per A-nonzero in LDS, then added to da once per chunk; dB is
+// scattered with global atomic adds.
 #include <algorithm>
 
 #include "device.hpp"
@@ -20,56 +27,84 @@ namespace {
 __device__ __forceinline__ void global_add(double *p, double v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void global_add(float *p, float v) { unsafeAtomicAdd(p, v); }
 
-// Products of one A row with per-A-nonzero reduction: f(b_index, a_value)
-// returns the dA contribution of that product; the sum per A-nonzero e is
-// written to da[e].  sda: U values of LDS owned by this unit.
-template <int U, typename V, typename F>
-__device__ __forceinline__ void reduce_products(int a_beg, int a_end, int unit, int nunits,
-                                                const int *__restrict__ acol, const V *__restrict__ aval,
-                                                const int *__restrict__ brpt, V *sda, V *__restrict__ da, F &&f)
-{
-    const int lane = lane_id();
-    const int gl = lane & (U - 1);
-    const int gbase = lane & ~(U - 1);
-    for (int c = a_beg + unit * U; c < a_end; c += nunits * U) {
-        const int e = c + gl;
-        int bs = 0, len = 0;
-        V av = V(0);
-        if (e < a_end) {
-            const int k = acol[e];
-            bs = brpt[k];
-            len = brpt[k + 1] - bs;
-            av = aval[e];
-        }
+// dA reduction of one unit: products add their dA term into the LDS slot of
+// their A-nonzero (q, 0..U-1); at the end of a chunk each lane moves its
+// slot into da[e].  The slots are zero between chunks.  When other units
+// walked slices of the same chunk the move is an atomic add; otherwise this
+// unit is the only one that ever handles e, so a plain read-modify-write is
+// enough (da starts at zero).
+template <int U, typename V>
+struct DaReduce {
+    V *sda;
+    V *__restrict__ da;
+    __device__ __forceinline__ void add(int q, V v) { atomicAdd(sda + q, v); }
+    __device__ __forceinline__ void flush(int e, bool valid, bool shared)
+    {
+        wave_lds_sync();
+        const int gl = lane_id() & (U - 1);
+        const V v = sda[gl];
         sda[gl] = V(0);
         wave_lds_sync();
-        const int incl = unit_incl_scan<U>(len);
-        const int excl = incl - len;
-        const int total = bperm(incl, gbase + U - 1);
-        for (int base = 0; base < total; base += U) {
-            const int p = base + gl;
-            int q = 0, qv = 0;
-#pragma unroll
-            for (int s = U >> 1; s >= 1; s >>= 1) {
-                const int cand = q + s;
-                const int v = bperm(excl, gbase + cand);
-                if (v <= p) {
-                    q = cand;
-                    qv = v;
-                }
-            }
-            const int qb = bperm(bs, gbase + q);
-            const V qa = bperm(av, gbase + q);
-            if (p < total) atomicAdd(sda + q, f(qb + (p - qv), qa));
+        if (valid && v != V(0)) {
+            if (shared)
+                global_add(da + e, v);
+            else
+                da[e] += v;
         }
-        wave_lds_sync();
-        if (e < a_end) da[e] = sda[gl];
-        wave_lds_sync();
     }
-}
+};
 
-// G lanes per row (G <= 64: BS/G rows per block; G == BS: one row per block).
-// Dynamic LDS: keys[RPB*T] | vals[RPB*T] | sda[BS].
+// Consumer for rows whose C row sits in an LDS hash table (kt keys, vt dC).
+template <int T, int H, int U, typename V>
+struct BwdHash {
+    const int *__restrict__ bcol;
+    const V *__restrict__ bval;
+    const int *kt;
+    const V *vt;
+    V *__restrict__ db;
+    DaReduce<U, V> red;
+    struct Item {
+        int k;
+        int c;
+        V b;
+    };
+    __device__ __forceinline__ Item load(int k) const { return Item{k, bcol[k], bval[k]}; }
+    __device__ __forceinline__ void apply(const Item &it, V a, int q)
+    {
+        const int h = hash_find<T, H>(kt, it.c);
+        const V g = h >= 0 ? vt[h] : V(0);
+        global_add(db + it.k, a * g);
+        red.add(q, g * it.b);
+    }
+    __device__ __forceinline__ void chunk_end(int e, bool valid, bool shared) { red.flush(e, valid, shared); }
+};
+
+// Consumer for one dense window [w0, w0 + W) of a long C row (g: dC by column - w0).
+template <int U, typename V>
+struct BwdWin {
+    const int *__restrict__ bcol;
+    const V *__restrict__ bval;
+    const V *g;
+    int w0;
+    V *__restrict__ db;
+    DaReduce<U, V> red;
+    struct Item {
+        int k;
+        int c;
+        V b;
+    };
+    __device__ __forceinline__ Item load(int k) const { return Item{k, bcol[k], bval[k]}; }
+    __device__ __forceinline__ void apply(const Item &it, V a, int q)
+    {
+        const V gv = g[it.c - w0];
+        global_add(db + it.k, a * gv);
+        red.add(q, gv * it.b);
+    }
+    __device__ __forceinline__ void chunk_end(int e, bool valid, bool shared) { red.flush(e, valid, shared); }
+};
+
+// G lanes per row (G <= 64: BS/G rows per block; G > 64: one row per block,
+// G/64 waves share the table).  Dynamic LDS: vals[RPB*T] | sda[BS] | keys[RPB*T].
 template <int G, int T, int BS, typename V>
 __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                  const V *__restrict__ aval, const int *__restrict__ brpt,
@@ -81,6 +116,7 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
     constexpr int U = G < kWave ? G : kWave;
     constexpr int RPB = BS / G;
     constexpr int NU = G / U;
+    constexpr int H = G <= kWave ? kHashLin : kHashFib;
     extern __shared__ unsigned long long smem_u64[];
     V *vals = reinterpret_cast<V *>(smem_u64);
     V *sda = vals + RPB * T;
@@ -90,6 +126,7 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
     const int slot = tid / G;
     const int unit = (tid % G) / U;
     for (int i = tid; i < RPB * T; i += BS) keys[i] = -1;
+    sda[tid] = V(0);
     __syncthreads();
     const int ridx = blockIdx.x * RPB + slot;
     int r = 0;
@@ -100,19 +137,15 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
         const long long c0 = crpt[r], c1 = crpt[r + 1];
         for (long long j = c0 + (tid % G); j < c1; j += G) {
             bool fresh;
-            const int h = hash_insert<T, (G <= kWave ? kHashLin : kHashFib)>(kt, ccol[j], fresh);
+            const int h = hash_insert<T, H>(kt, ccol[j], fresh);
             vt[h] = dc[j];
         }
     }
     __syncthreads();
     if (ridx >= nrows) return;
-    reduce_products<U, V>(arpt[r], arpt[r + 1], unit, NU, acol, aval, brpt, sda + (tid / U) * U, da,
-                          [&](int k, V a) -> V {
-                              const int h = hash_find<T, (G <= kWave ? kHashLin : kHashFib)>(kt, bcol[k]);
-                              const V g = h >= 0 ? vt[h] : V(0);
-                              global_add(db + k, a * g);
-                              return g * bval[k];
-                          });
+    BwdHash<T, H, U, V> op{bcol, bval, kt, vt, db, DaReduce<U, V>{sda + (tid / U) * U, da}};
+    const FullRow range{brpt};
+    for_each_product<U, true, V>(arpt[r], arpt[r + 1], unit, NU, acol, aval, range, op);
 }
 
 constexpr int kLongBS = 1024;
@@ -131,38 +164,71 @@ __global__ void k_zero_vals_f32(float *p, long long n)
         p[i] = 0.f;
 }
 
-// Rows of C above 4096 entries: binary search of the sorted C row.
+// Rows of C above 4096 entries: dense dC windows of W columns (dynamic LDS
+// of W values).  cursor[] holds two ints per A-nonzero index (span = highest
+// A-nonzero index + 1); windows alternate which half they read and which
+// they write, so waves that share an A chunk never read a position another
+// wave has already advanced.
 template <typename V>
-__global__ __launch_bounds__(kLongBS) void k_bwd_long(const int *__restrict__ arpt, const int *__restrict__ acol,
-                                                      const V *__restrict__ aval, const int *__restrict__ brpt,
-                                                      const int *__restrict__ bcol, const V *__restrict__ bval,
-                                                      const int *__restrict__ rows, int nrows,
-                                                      const long long *__restrict__ crpt,
-                                                      const int *__restrict__ ccol, const V *__restrict__ dc,
-                                                      V *__restrict__ da, V *__restrict__ db)
+__global__ __launch_bounds__(kLongBS) void k_bwd_win(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                     const V *__restrict__ aval, const int *__restrict__ brpt,
+                                                     const int *__restrict__ bcol, const V *__restrict__ bval,
+                                                     const int *__restrict__ rows, int nrows, int ncols,
+                                                     const long long *__restrict__ crpt,
+                                                     const int *__restrict__ ccol, const V *__restrict__ dc,
+                                                     V *__restrict__ da, V *__restrict__ db, int *__restrict__ cursor,
+                                                     int span)
 {
+    using D = DenseWin<V>;
+    extern __shared__ unsigned long long smem_u64[];
+    V *g = reinterpret_cast<V *>(smem_u64);
     __shared__ V sda[kLongBS];
+    __shared__ int s_cnt[kLongBS / kWave];
     constexpr int NU = kLongBS / kWave;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
+    sda[tid] = V(0);
+    __syncthreads();
+
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
         const int r = rows ? rows[ridx] : ridx;
-        const long long c0 = crpt[r], c1 = crpt[r + 1];
-        reduce_products<kWave, V>(arpt[r], arpt[r + 1], wave, NU, acol, aval, brpt, sda + wave * kWave, da,
-                                  [&](int k, V a) -> V {
-                                      const int col = bcol[k];
-                                      long long lo = c0, hi = c1;
-                                      while (lo < hi) {
-                                          const long long m = (lo + hi) >> 1;
-                                          if (ccol[m] < col)
-                                              lo = m + 1;
-                                          else
-                                              hi = m;
-                                      }
-                                      const V g = (lo < c1 && ccol[lo] == col) ? dc[lo] : V(0);
-                                      global_add(db + k, a * g);
-                                      return g * bval[k];
-                                  });
+        const int a_beg = arpt[r], a_end = arpt[r + 1];
+        const long long c1 = crpt[r + 1];
+        long long cpos = crpt[r];
+        bool first = true;
+        int parity = 0;
+        while (cpos < c1) {
+            // Window of the next column of row i of C; every product of this
+            // window has its column among the staged entries.
+            const int w0 = (ccol[cpos] / D::W) * D::W;
+            const int wend = (ncols - w0 <= D::W) ? ncols : w0 + D::W;
+            // The row's entries below wend are a prefix of [cpos, c1) of at
+            // most W entries (distinct columns of the window).
+            const long long lim = min(c1, cpos + static_cast<long long>(D::W));
+            int cnt = 0;
+            for (long long j = cpos + tid; j < lim; j += kLongBS) {
+                const int c = ccol[j];
+                if (c < wend) {
+                    g[c - w0] = dc[j];
+                    cnt++;
+                }
+            }
+            cnt = unit_sum<kWave>(cnt);
+            if (lane_id() == 0) s_cnt[wave] = cnt;
+            __syncthreads();
+            int tot = 0;
+            for (int w = 0; w < NU; w++) tot += s_cnt[w];
+            int nxt = INT_MAX;
+            const CursorRange range{brpt, bcol, cursor + (parity ? span : 0), cursor + (parity ? 0 : span), wend,
+                                    first, wend >= ncols, &nxt};
+            BwdWin<kWave, V> op{bcol, bval, g, w0, db, DaReduce<kWave, V>{sda + wave * kWave, da}};
+            for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op);
+            // Every read of g and s_cnt of this window is done before the next one stages.
+            __syncthreads();
+            cpos += tot;
+            first = false;
+            parity ^= 1;
+        }
     }
 }
 
@@ -200,6 +266,8 @@ void set_attrs()
     set_attr<64, 512, 256, V>();
     set_attr<256, 2048, 256, V>();
     set_attr<1024, 8192, 1024, V>();
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_bwd_win<V>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(sizeof(V) * DenseWin<V>::W));
 }
 
 }  // namespace
@@ -212,7 +280,7 @@ void backward_init_attributes()
 
 template <typename V>
 void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                         const int *ccol, const V *dc, V *da, V *db, hipStream_t s)
+                         const int *ccol, const V *dc, V *da, V *db, int *cursor, int span, hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
@@ -224,17 +292,17 @@ void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *row
         case 5: launch_hash<256, 2048, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
         case 6: launch_hash<1024, 8192, 1024, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
         default:
-            k_bwd_long<V><<<nrows, kLongBS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col,
-                                                    static_cast<const V *>(B.val), rows, nrows, crpt, ccol, dc, da,
-                                                    db);
+            k_bwd_win<V><<<nrows, kLongBS, sizeof(V) * DenseWin<V>::W, s>>>(
+                A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
+                B.N, crpt, ccol, dc, da, db, cursor, span);
             break;
     }
 }
 
 template void launch_backward_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *,
-                                          const int *, const double *, double *, double *, hipStream_t);
+                                          const int *, const double *, double *, double *, int *, int, hipStream_t);
 template void launch_backward_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *,
-                                         const int *, const float *, float *, float *, hipStream_t);
+                                         const int *, const float *, float *, float *, int *, int, hipStream_t);
 
 void launch_zero_values(void *p, long long n, bool f32, hipStream_t s)
 {

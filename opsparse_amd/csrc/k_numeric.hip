@@ -35,7 +35,7 @@ struct NumInsert {
         V b;
     };
     __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
-    __device__ __forceinline__ void apply(const Item &it, V a)
+    __device__ __forceinline__ void apply(const Item &it, V a, int)
     {
         bool fresh;
         const int h = hash_insert<T, H>(kt, it.c, fresh);
@@ -55,7 +55,7 @@ struct DenseInsert {
         V b;
     };
     __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
-    __device__ __forceinline__ void apply(const Item &it, V a)
+    __device__ __forceinline__ void apply(const Item &it, V a, int)
     {
         const int c = it.c - w0;
         atomicAdd(acc + c, a * it.b);
@@ -217,58 +217,6 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
 }
 
 constexpr int kLongBS = 1024;
-
-template <typename V>
-struct DenseWin {
-    static constexpr int W = sizeof(V) == 8 ? 16384 : 32768;  // columns per window: 128 KB of values
-    static constexpr int WORDS = W / 32;
-    static constexpr size_t lds = sizeof(V) * W + sizeof(unsigned) * WORDS;
-};
-
-// Cursor-carrying B range for window [w0, wend): lo is the first entry of
-// the B row not consumed by earlier windows; hi the first entry >= wend,
-// found by galloping from lo.  Tracks the smallest column beyond the window
-// so empty windows can be skipped.
-struct CursorRange {
-    const int *__restrict__ brpt;
-    const int *__restrict__ bcol;
-    const int *__restrict__ cur_in;  // position after the previous window (unused on the first)
-    int *__restrict__ cur_out;       // position after this window
-    int wend;
-    bool first;
-    bool last;  // wend covers all columns
-    int *next_col;
-    __device__ __forceinline__ void operator()(int e, int k, int &bs, int &len) const
-    {
-        const int t = brpt[k + 1];
-        const int lo = first ? brpt[k] : cur_in[e];
-        int hi;
-        if (last) {
-            hi = t;
-        } else if (lo >= t || bcol[lo] >= wend) {
-            hi = lo;
-        } else {
-            int prev = lo, step = 1;
-            while (prev + step < t && bcol[prev + step] < wend) {
-                prev += step;
-                step <<= 1;
-            }
-            int l = prev + 1, h = min(prev + step, t);
-            while (l < h) {
-                const int m = (l + h) >> 1;
-                if (bcol[m] < wend)
-                    l = m + 1;
-                else
-                    h = m;
-            }
-            hi = l;
-        }
-        cur_out[e] = hi;
-        if (hi < t) *next_col = min(*next_col, bcol[hi]);
-        bs = lo;
-        len = hi - lo;
-    }
-};
 
 // Bin 7: rows above 4096 entries.  Dense LDS accumulator + occupancy bitmap
 // per window of W columns; windows are written out in column order, so the

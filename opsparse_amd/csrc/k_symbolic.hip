@@ -30,7 +30,7 @@ struct SymInsert {
     int cnt = 0;
     using Item = int;
     __device__ __forceinline__ int load(int k) const { return bcol[k]; }
-    __device__ __forceinline__ void apply(int c, double)
+    __device__ __forceinline__ void apply(int c, double, int)
     {
         bool fresh;
         hash_insert<T, H>(tab, c, fresh);
@@ -45,7 +45,7 @@ struct BitmapInsert {
     int cnt = 0;
     using Item = int;
     __device__ __forceinline__ int load(int k) const { return bcol[k]; }
-    __device__ __forceinline__ void apply(int col, double)
+    __device__ __forceinline__ void apply(int col, double, int)
     {
         const int c = col - w0;
         const unsigned bit = 1u << (c & 31);

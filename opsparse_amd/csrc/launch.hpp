@@ -116,7 +116,7 @@ void symbolic_init_attributes();
 // --- backward (k_backward.hip) ---
 template <typename V>
 void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                         const int *ccol, const V *dc, V *da, V *db, hipStream_t s);
+                         const int *ccol, const V *dc, V *da, V *db, int *cursor, int span, hipStream_t s);
 void backward_init_attributes();
 void launch_zero_values(void *p, long long n, bool f32, hipStream_t s);
 // row_nnz[i] = crpt[i+1] - crpt[i] (numeric binning key for the backward).

@@ -22,6 +22,7 @@
 
 #include "launch.hpp"
 #include "opsparse_amd.h"
+#include "outmem.hpp"
 
 using namespace opsamd;
 
@@ -370,13 +371,13 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
         *C64 = ops_csr64{};
         C64->M = M;
         C64->N = B->N;
-        CK(hipMalloc(&C64->rpt, sizeof(int64_t) * (static_cast<size_t>(M) + 1)));
+        CK(out_malloc(reinterpret_cast<void **>(&C64->rpt), sizeof(int64_t) * (static_cast<size_t>(M) + 1)));
         crpt64_dst = reinterpret_cast<long long *>(C64->rpt);
     } else {
         *C32 = ops_csr{};
         C32->M = M;
         C32->N = B->N;
-        CK(hipMalloc(&C32->rpt, sizeof(int32_t) * (static_cast<size_t>(M) + 1)));
+        CK(out_malloc(reinterpret_cast<void **>(&C32->rpt), sizeof(int32_t) * (static_cast<size_t>(M) + 1)));
         ws->crpt64.ensure(static_cast<size_t>(M) + 1);
         crpt64_dst = ws->crpt64.p;
         crpt32_dst = C32->rpt;
@@ -393,8 +394,8 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
     const size_t nz = static_cast<size_t>(std::max<long long>(pc.nnz, 1));
     int *ccol = nullptr;
     V *cval = nullptr;
-    CK(hipMalloc(&ccol, nz * sizeof(int)));
-    CK(hipMalloc(&cval, nz * sizeof(V)));
+    CK(out_malloc(reinterpret_cast<void **>(&ccol), nz * sizeof(int)));
+    CK(out_malloc(reinterpret_cast<void **>(&cval), nz * sizeof(V)));
     if (C64) {
         C64->nnz = pc.nnz;
         C64->col = ccol;
@@ -468,9 +469,18 @@ void backward_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const 
         CK(hipGetLastError());
         rows = ws->bins.p;
     }
+    // Rows of C above the largest LDS table walk dense column windows with
+    // per-A-nonzero cursors (two ints per A-nonzero index, double-buffered).
+    int *cursor = nullptr;
+    int span = 0;
+    if (st.num_size[kNumBins - 1] > 0) {
+        span = static_cast<int>(a_index_span(A, s));
+        ws->cursor.ensure(2 * static_cast<size_t>(span) + 2);
+        cursor = ws->cursor.p;
+    }
     dispatch_bins(ws, s, st.num_size, false, [&](int bin, hipStream_t ps) {
         launch_backward_bin<V>(bin, a, b, rows ? rows + off[bin] : nullptr, st.num_size[bin], crpt, ccol, dc, da, db,
-                               ps);
+                               cursor, span, ps);
     });
 }
 
@@ -588,9 +598,9 @@ int ops_workspace_get_tuning(const ops_workspace *ws, ops_tuning *t)
 int ops_csr_free(ops_csr *C)
 {
     if (!C) return OPS_INVALID;
-    if (C->rpt) (void)hipFree(C->rpt);
-    if (C->col) (void)hipFree(C->col);
-    if (C->val) (void)hipFree(C->val);
+    if (C->rpt) (void)out_free(C->rpt);
+    if (C->col) (void)out_free(C->col);
+    if (C->val) (void)out_free(C->val);
     *C = ops_csr{};
     return OPS_OK;
 }
@@ -598,9 +608,9 @@ int ops_csr_free(ops_csr *C)
 int ops_csr64_free(ops_csr64 *C)
 {
     if (!C) return OPS_INVALID;
-    if (C->rpt) (void)hipFree(C->rpt);
-    if (C->col) (void)hipFree(C->col);
-    if (C->val) (void)hipFree(C->val);
+    if (C->rpt) (void)out_free(C->rpt);
+    if (C->col) (void)out_free(C->col);
+    if (C->val) (void)out_free(C->val);
     *C = ops_csr64{};
     return OPS_OK;
 }

@@ -18,7 +18,8 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from ._lib import (OPS_F32, OPS_F64, OPS_OK, OpsCsr, OpsCsr64, OpsError, OpsTimings, OpsTuning, check, lib)
+from ._lib import (OPS_F32, OPS_F64, OPS_OK, OpsCsr, OpsCsr64, OpsError, OpsMemStats, OpsTimings, OpsTuning, check,
+                   lib)
 from .gen import HostCSR
 
 _HIP = None
@@ -109,6 +110,23 @@ class Workspace:
             pass
 
 
+def mem_cache_stats() -> dict:
+    """Output-memory cache counters (ops_mem_cache_stats)."""
+    st = OpsMemStats()
+    check(lib().ops_mem_cache_stats(C.byref(st)))
+    return st.as_dict()
+
+
+def set_mem_cache_limit(nbytes: int) -> None:
+    """Caps the bytes the output cache keeps per device (< 0: default, 0: off)."""
+    check(lib().ops_mem_cache_set_limit(int(nbytes)))
+
+
+def mem_cache_release() -> None:
+    """Frees every cached output block."""
+    check(lib().ops_mem_cache_release())
+
+
 def default_tuning() -> dict:
     t = OpsTuning()
     check(lib().ops_tuning_default(C.byref(t)))
@@ -167,6 +185,29 @@ class LibCSR:
                 if st != 0:
                     raise RuntimeError(f"hipMemcpy failed: {st}")
         return rpt.astype(np.int64), col, val
+
+    def rows_to_host(self, r0: int, r1: int):
+        """Rows [r0, r1) as (rpt rebased to 0, int64; col; val) numpy copies,
+        reading only that slice of the device arrays."""
+        if not 0 <= r0 <= r1 <= self.M:
+            raise ValueError(f"row range [{r0}, {r1}) outside [0, {self.M}]")
+        hip = _hip()
+        ridx = np.int64 if self.wide else np.int32
+        rpt = np.empty(r1 - r0 + 1, dtype=ridx)
+        torch.cuda.synchronize()
+
+        def copy(dst, src, nbytes):
+            if nbytes and hip.hipMemcpy(dst.ctypes.data, src, nbytes, 2) != 0:
+                raise RuntimeError("hipMemcpy failed")
+
+        copy(rpt, self.d.rpt + r0 * rpt.itemsize, rpt.nbytes)
+        rpt = rpt.astype(np.int64)
+        s0, s1 = int(rpt[0]), int(rpt[-1])
+        col = np.empty(s1 - s0, dtype=np.int32)
+        val = np.empty(s1 - s0, dtype=np.float64 if self.dtype == torch.float64 else np.float32)
+        copy(col, self.d.col + s0 * 4, col.nbytes)
+        copy(val, self.d.val + s0 * val.itemsize, val.nbytes)
+        return rpt - s0, col, val
 
     def desc(self) -> OpsCsr:
         """Input descriptor of this (int32-offset) result, e.g. a packed batch."""
@@ -338,4 +379,4 @@ def spgemm_autograd(a_val: torch.Tensor, b_val: torch.Tensor, A: DeviceCSR, B: D
 
 __all__ = ["DeviceCSR", "Workspace", "LibCSR", "Plan", "spgemm", "spgemm_safe", "backward", "SpGEMMFunction",
            "spgemm_autograd", "default_workspace", "OpsTimings", "OpsError", "dtype_code", "batch_pack",
-           "default_tuning", "tuning_caps"]
+           "default_tuning", "tuning_caps", "mem_cache_stats", "set_mem_cache_limit", "mem_cache_release"]

@@ -102,3 +102,11 @@ def test_tuning_defaults_within_caps_without_gpu():
         if i:
             assert d.sym_bound[i] >= d.sym_bound[i - 1] and d.num_bound[i] >= d.num_bound[i - 1]
     assert _lib.lib().ops_workspace_set_tuning(None, C.byref(d)) == _lib.OPS_INVALID
+
+
+def test_mem_cache_counters_without_gpu():
+    st = _lib.OpsMemStats()
+    assert _lib.lib().ops_mem_cache_stats(C.byref(st)) == _lib.OPS_OK
+    assert st.cached_bytes == 0 and st.live_bytes == 0 and st.limit == -1
+    assert _lib.lib().ops_mem_cache_stats(None) == _lib.OPS_INVALID
+    assert _lib.lib().ops_free(None) == _lib.OPS_OK  # like hipFree(NULL)

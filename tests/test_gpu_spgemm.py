@@ -11,7 +11,8 @@ import torch
 
 import oracle
 import opsparse_amd.gen as G
-from opsparse_amd.spgemm import DeviceCSR, Plan, spgemm
+from opsparse_amd.spgemm import (DeviceCSR, Plan, mem_cache_release, mem_cache_stats, set_mem_cache_limit,
+                                  spgemm)
 from helpers import TOL, check_product, csr_from_dense, csr_from_rows, random_csr
 
 pytestmark = pytest.mark.gpu
@@ -117,6 +118,21 @@ def test_wide_rowptr(gpu):
     run(A, A, wide=True)
 
 
+@pytest.mark.parametrize("wide", [False, True])
+def test_rows_to_host_matches_full_copy(gpu, wide):
+    A = G.rmat(10, 8, 6)
+    dA = DeviceCSR.from_host(A, torch.float64)
+    C = spgemm(dA, dA, wide=wide)
+    rpt, col, val = C.to_host()
+    for r0, r1 in ((0, 1), (0, A.M), (17, 300), (A.M - 5, A.M), (40, 40)):
+        hr, hc, hv = C.rows_to_host(r0, r1)
+        s0, s1 = rpt[r0], rpt[r1]
+        assert np.array_equal(hr, rpt[r0:r1 + 1] - s0)
+        assert np.array_equal(hc, col[s0:s1]) and np.array_equal(hv, val[s0:s1])
+    with pytest.raises(ValueError):
+        C.rows_to_host(5, A.M + 1)
+
+
 def test_plan_and_graph(gpu):
     A = G.rmat(12, 12, 9)
     dA = DeviceCSR.from_host(A, torch.float64)
@@ -139,3 +155,26 @@ def test_plan_and_graph(gpu):
     p.numeric(dA, dA, col_g, val_g, graph=True)
     torch.cuda.synchronize()
     oracle.compare(rpt, col_g.cpu().numpy(), val_g.cpu().numpy(), oracle.spgemm(A2, A2), TOL[np.float64])
+
+
+def test_output_cache_reuse_and_limit(gpu):
+    A = G.rmat(12, 10, 8)
+    dA = DeviceCSR.from_host(A, torch.float64)
+    ref = oracle.spgemm(A, A)
+    mem_cache_release()
+    before = mem_cache_stats()
+    for wide in (False, False, True, True):
+        C = spgemm(dA, dA, wide=wide)
+        rpt, col, val = C.to_host()
+        oracle.compare(rpt, col, val, ref, TOL[np.float64])
+        C.free()
+    st = mem_cache_stats()
+    assert st["hits"] - before["hits"] >= 4  # the second call of each width reuses col, val (and rpt)
+    assert st["cached_bytes"] > 0 and st["live_bytes"] <= before["live_bytes"]
+    set_mem_cache_limit(0)  # caching off: cached blocks are freed, releases go to hipFree
+    assert mem_cache_stats()["cached_bytes"] == 0
+    C = spgemm(dA, dA)
+    C.free()
+    assert mem_cache_stats()["cached_bytes"] == 0
+    set_mem_cache_limit(-1)
+    mem_cache_release()
