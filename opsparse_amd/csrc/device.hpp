@@ -149,10 +149,43 @@ template <typename Op>
 struct has_chunk_end<Op, std::void_t<decltype(&Op::chunk_end)>> : std::true_type {
 };
 
+// Instead of apply, a consumer may split its update into
+//   Slot prep(const Item &);                          // LDS reads only
+//   void commit(const Item &, const Slot &, V a, int q);  // the updates
+// The walkers then issue the reads of every product of a step before any
+// update.  LDS operations of a wave complete in order, so with a plain apply
+// the read of each product waits for the atomics of the products before it;
+// batched, the reads and atomics of a step overlap.
+template <typename Op, typename = void>
+struct has_prep : std::false_type {
+};
+template <typename Op>
+struct has_prep<Op, std::void_t<decltype(&Op::prep)>> : std::true_type {
+};
+
+template <int NP, typename Op, typename V>
+__device__ __forceinline__ void apply_batch(Op &op, const typename Op::Item (&it)[NP], const V (&a)[NP],
+                                            const int (&q)[NP], const bool (&ok)[NP])
+{
+    if constexpr (has_prep<Op>::value) {
+        decltype(op.prep(it[0])) sl[NP];
+#pragma unroll
+        for (int n = 0; n < NP; n++)
+            if (ok[n]) sl[n] = op.prep(it[n]);
+#pragma unroll
+        for (int n = 0; n < NP; n++)
+            if (ok[n]) op.commit(it[n], sl[n], a[n], q[n]);
+    } else {
+#pragma unroll
+        for (int n = 0; n < NP; n++)
+            if (ok[n]) op.apply(it[n], a[n], q[n]);
+    }
+}
+
 // Grouped walk of one loaded chunk: GS lanes per A-nonzero, each group
 // strides over its B row two entries at a time.  No per-product cross-lane
 // search; used when the chunk's B rows have similar lengths.
-template <int U, int GS, bool WITH_A, typename V, typename Op>
+template <int U, int GS, bool WITH_A, typename V, bool SLICE, typename Op>
 __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op &op, int part, int nparts)
 {
     static_assert(GS <= U, "group wider than unit");
@@ -162,7 +195,16 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
     constexpr int PER = U / GS;
     const int sub = gl & (GS - 1);
     const int grp = gl / GS;
-    for (int q0 = part * PER; q0 < cnt; q0 += nparts * PER) {
+    // Several parts (units sharing the chunk) split it by A-nonzero groups
+    // (part p takes rounds p, p+nparts, ...) or, with SLICE, by slicing
+    // every B row (part p takes steps p, p+nparts, ... of each row).
+    // Slicing keeps all parts busy on rows with few A-nonzeros and long B
+    // rows; group distribution suits many A-nonzeros with short B rows.
+    const int t0 = SLICE ? sub + part * 2 * GS : sub;
+    const int tstep = SLICE ? nparts * 2 * GS : 2 * GS;
+    const int qstart = SLICE ? 0 : part * PER;
+    const int qstep = SLICE ? PER : nparts * PER;
+    for (int q0 = qstart; q0 < cnt; q0 += qstep) {
         const int q = q0 + grp;
         const int src = gbase + (q < U ? q : 0);
         // Every bpermute runs with the whole unit active: a source lane that
@@ -172,13 +214,14 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
         const int ql = q < cnt ? lq : 0;
         V qa = V(0);
         if constexpr (WITH_A) qa = bperm(av, src);
-        for (int t = sub; t < ql; t += 2 * GS) {
-            const bool two = t + GS < ql;
-            const typename Op::Item i0 = op.load(qb + t);
-            typename Op::Item i1{};
-            if (two) i1 = op.load(qb + t + GS);
-            op.apply(i0, qa, q);
-            if (two) op.apply(i1, qa, q);
+        for (int t = t0; t < ql; t += tstep) {
+            typename Op::Item it[2];
+            const bool ok[2] = {true, t + GS < ql};
+            it[0] = op.load(qb + t);
+            if (ok[1]) it[1] = op.load(qb + t + GS);
+            const V aa[2] = {qa, qa};
+            const int qq[2] = {q, q};
+            apply_batch<2>(op, it, aa, qq, ok);
         }
     }
 }
@@ -230,12 +273,118 @@ __device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op, int pa
             a0 = bperm(av, gbase + q0);
             a1 = bperm(av, gbase + q1);
         }
-        const bool ok0 = p0 < total, ok1 = p1 < total;
-        typename Op::Item i0{}, i1{};
-        if (ok0) i0 = op.load(b0 + (p0 - v0));
-        if (ok1) i1 = op.load(b1 + (p1 - v1));
-        if (ok0) op.apply(i0, a0, q0);
-        if (ok1) op.apply(i1, a1, q1);
+        const bool ok[2] = {p0 < total, p1 < total};
+        typename Op::Item it[2];
+        if (ok[0]) it[0] = op.load(b0 + (p0 - v0));
+        if (ok[1]) it[1] = op.load(b1 + (p1 - v1));
+        const V aa[2] = {a0, a1};
+        const int qq[2] = {q0, q1};
+        apply_batch<2>(op, it, aa, qq, ok);
+    }
+}
+
+// Per-unit LDS staging of a chunk for the staged flat walk.
+template <int U, typename V>
+struct FlatStage {
+    int excl[U + 1];
+    int bs[U];
+    V av[U];
+};
+
+// Flattened walk with the chunk staged in LDS: the scanned B-range offsets,
+// B starts and A values of the chunk's A-nonzeros go to the unit's LDS
+// stage, and each lane walks its products (l, l+U, ... of this part's
+// steps, NP per step) with a running A-nonzero index that only moves
+// forward: one LDS read per product while it stays in the same B range, a
+// binary search over the staged offsets when it leaves it.  Compared with
+// chunk_flat this replaces a log2(U)-deep chain of cross-lane reads per
+// product by one LDS read, and keeps the NP loads of the next step in flight
+// during this step's updates; it pays off on chunks of long, skewed B
+// ranges (power-law rows).
+template <int U, bool WITH_A, typename V, typename Op>
+__device__ __forceinline__ void chunk_flat_staged(int bs, int len, V av, Op &op, int part, int nparts,
+                                                  FlatStage<U, V> &st)
+{
+    constexpr int NP = 4;
+    const int lane = lane_id();
+    const int gl = lane & (U - 1);
+    const int gbase = lane & ~(U - 1);
+    const int incl = unit_incl_scan<U>(len);
+    const int total = bperm(incl, gbase + U - 1);
+    wave_lds_sync();  // the previous chunk's stage is no longer read
+    st.excl[gl] = incl - len;
+    st.bs[gl] = bs;
+    if constexpr (WITH_A) st.av[gl] = av;
+    if (gl == U - 1) st.excl[U] = total;
+    wave_lds_sync();
+    int q = 0;
+    int q_lo = 0, q_hi = st.excl[1], q_bs = st.bs[0];
+    V q_av = WITH_A ? st.av[0] : V(0);
+    // Positions of step `base`: B index, A value and A-nonzero of each product.
+    auto gather = [&](int base, int (&bb)[NP], V (&aa)[NP], int (&qq)[NP], bool (&ok)[NP]) {
+#pragma unroll
+        for (int n = 0; n < NP; n++) {
+            const int p = base + n * U + gl;
+            ok[n] = p < total;
+            if (ok[n]) {
+                if (p >= q_hi) {
+                    // Largest q' > q with excl[q'] <= p (excl[U] = total > p).
+                    int lo = q + 1, hi = U - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (st.excl[mid] <= p)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    q = lo;
+                    q_lo = st.excl[q];
+                    q_hi = st.excl[q + 1];
+                    q_bs = st.bs[q];
+                    if constexpr (WITH_A) q_av = st.av[q];
+                }
+                bb[n] = q_bs + (p - q_lo);
+                aa[n] = q_av;
+                qq[n] = q;
+            }
+        }
+    };
+    // Software pipeline: the B loads of the next step are in flight while
+    // this step's updates run.
+    const int stride = nparts * NP * U;
+    int base = part * NP * U;
+    if (base >= total) return;
+    int bb[NP], qq[NP];
+    V aa[NP];
+    bool ok[NP];
+    typename Op::Item it[NP];
+    gather(base, bb, aa, qq, ok);
+#pragma unroll
+    for (int n = 0; n < NP; n++)
+        if (ok[n]) it[n] = op.load(bb[n]);
+    while (true) {
+        const int nb = base + stride;
+        const bool more = nb < total;
+        int bb2[NP], qq2[NP];
+        V aa2[NP];
+        bool ok2[NP];
+        typename Op::Item it2[NP];
+        if (more) {
+            gather(nb, bb2, aa2, qq2, ok2);
+#pragma unroll
+            for (int n = 0; n < NP; n++)
+                if (ok2[n]) it2[n] = op.load(bb2[n]);
+        }
+        apply_batch<NP>(op, it, aa, qq, ok);
+        if (!more) break;
+        base = nb;
+#pragma unroll
+        for (int n = 0; n < NP; n++) {
+            it[n] = it2[n];
+            aa[n] = aa2[n];
+            qq[n] = qq2[n];
+            ok[n] = ok2[n];
+        }
     }
 }
 
@@ -253,11 +402,13 @@ __device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op, int pa
 // Per chunk of U A-nonzeros the walk is chosen at run time: if the longest
 // B row is at most twice the mean, lanes are grouped per A-nonzero (group
 // width about half the mean, 4..U); otherwise products are flattened
-// across the unit.
-template <int U, bool WITH_A, typename V, typename Range, typename Op>
+// across the unit (staged in the unit's LDS `stage` when one is given).
+// Units that share a chunk always take the staged flat walk when a stage
+// is given.
+template <int U, bool WITH_A, typename V, bool SLICE = false, typename Range, typename Op>
 __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit, int nunits,
                                                  const int *__restrict__ acol, const V *__restrict__ aval,
-                                                 const Range &range, Op &op)
+                                                 const Range &range, Op &op, FlatStage<U, V> *stage = nullptr)
 {
     const int lane = lane_id();
     const int gl = lane & (U - 1);
@@ -280,22 +431,29 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
         const int mx = unit_max<U>(len);
         if (total == 0) continue;
         const int mean = (total + cnt - 1) / cnt;
-        if (mx <= 2 * mean) {
+        if (stage && shared) {
+            // Units sharing a chunk: each takes steps of NP*U consecutive
+            // products across A-nonzero boundaries, instead of one A-nonzero
+            // (two products per lane) per round trip.
+            chunk_flat_staged<U, WITH_A, V>(bs, len, av, op, part, nparts, *stage);
+        } else if (mx <= 2 * mean) {
             // Groups of about half the mean length: several products per lane per
             // round amortise the per-round cross-lane reads.
             const int half = mean >> 1;
             const int gs = half >= 64 ? 64 : (half >= 32 ? 32 : (half >= 16 ? 16 : (half >= 8 ? 8 : 4)));
             if (gs <= 4) {
-                chunk_grouped<U, 4, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
+                chunk_grouped<U, 4, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
             } else if (gs == 8 || U == 8) {
-                chunk_grouped<U, 8, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
+                chunk_grouped<U, 8, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
             } else if (gs == 16 || U == 16) {
-                if constexpr (U >= 16) chunk_grouped<U, 16, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
+                if constexpr (U >= 16) chunk_grouped<U, 16, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
             } else if (gs == 32 || U == 32) {
-                if constexpr (U >= 32) chunk_grouped<U, 32, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
+                if constexpr (U >= 32) chunk_grouped<U, 32, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
             } else {
-                if constexpr (U >= 64) chunk_grouped<U, 64, WITH_A, V>(cnt, bs, len, av, op, part, nparts);
+                if constexpr (U >= 64) chunk_grouped<U, 64, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
             }
+        } else if (stage) {
+            chunk_flat_staged<U, WITH_A, V>(bs, len, av, op, part, nparts, *stage);
         } else {
             chunk_flat<U, WITH_A, V>(bs, len, av, op, part, nparts);
         }

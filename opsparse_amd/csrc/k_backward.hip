@@ -94,9 +94,9 @@ struct BwdWin {
         V b;
     };
     __device__ __forceinline__ Item load(int k) const { return Item{k, bcol[k], bval[k]}; }
-    __device__ __forceinline__ void apply(const Item &it, V a, int q)
+    __device__ __forceinline__ V prep(const Item &it) const { return g[it.c - w0]; }
+    __device__ __forceinline__ void commit(const Item &it, const V &gv, V a, int q)
     {
-        const V gv = g[it.c - w0];
         global_add(db + it.k, a * gv);
         red.add(q, gv * it.b);
     }
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
     if (ridx >= nrows) return;
     BwdHash<T, H, U, V> op{bcol, bval, kt, vt, db, DaReduce<U, V>{sda + (tid / U) * U, da}};
     const FullRow range{brpt};
-    for_each_product<U, true, V>(arpt[r], arpt[r + 1], unit, NU, acol, aval, range, op);
+    for_each_product<U, true, V, (G > 256)>(arpt[r], arpt[r + 1], unit, NU, acol, aval, range, op);
 }
 
 constexpr int kLongBS = 1024;
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_win(const int *__restrict__ arp
             const CursorRange range{brpt, bcol, cursor + (parity ? span : 0), cursor + (parity ? 0 : span), wend,
                                     first, wend >= ncols, &nxt};
             BwdWin<kWave, V> op{bcol, bval, g, w0, db, DaReduce<kWave, V>{sda + wave * kWave, da}};
-            for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op);
+            for_each_product<kWave, true, V, true>(a_beg, a_end, wave, NU, acol, aval, range, op);
             // Every read of g and s_cnt of this window is done before the next one stages.
             __syncthreads();
             cpos += tot;

@@ -6,15 +6,17 @@
 // (include/opsparse/core_algo/spgemm/kernels/numeric.cuh:9-393) and their
 // dispatch h_numeric_full_occu (src/opsparse/core_algo/spgemm/spgemm_phases.cu:148-206).
 //
-// The accumulation is the same open-addressing hash (CAS on the key, atomic
-// add of a*b on the value), with tables of >= 2n slots for a row of n
-// entries.  The ordering step is different: the reference ranks every entry
-// against every other entry, O(n^2) (numeric.cuh:97-106, 194-204, 382-392);
-// here rows of up to 256 entries are compacted and bitonic-sorted in
-// registers across their unit (cross-lane ds_bpermute), rows up to 4096 are
-// bitonic-sorted in LDS, and rows above 4096 entries never need a sort: they
-// accumulate into dense LDS column windows whose occupancy bitmap is read
-// out in column order.
+// Rows up to 4096 entries accumulate in an open-addressing LDS hash (CAS on
+// the key, atomic add of a*b on the value), with tables of >= 2n slots for
+// a row of n entries, like the reference.  The ordering step is different:
+// the reference ranks every entry against every other entry, O(n^2)
+// (numeric.cuh:97-106, 194-204, 382-392); here rows of up to 256 entries
+// are compacted and bitonic-sorted in registers across their unit
+// (cross-lane ds_bpermute) and rows up to 4096 are bitonic-sorted in LDS.
+// Rows above 4096 entries use no hash and no sort: a bitmap of the row's
+// columns per 2^18-column window gives every column its rank (its position
+// in the sorted output), values accumulate at that rank in LDS, and the
+// window is written out in order (k_num_rank).
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -22,8 +24,8 @@ namespace opsamd {
 
 namespace {
 
-// Consumers of the product walk (device.hpp): accumulate a*b into the
-// row's hash table, or into a dense column window.
+// Consumer of the product walk (device.hpp): accumulate a*b into the row's
+// hash table.
 template <int T, int H, typename V>
 struct NumInsert {
     const int *__restrict__ bcol;
@@ -40,27 +42,6 @@ struct NumInsert {
         bool fresh;
         const int h = hash_insert<T, H>(kt, it.c, fresh);
         atomicAdd(vt + h, a * it.b);
-    }
-};
-
-template <typename V>
-struct DenseInsert {
-    const int *__restrict__ bcol;
-    const V *__restrict__ bval;
-    V *acc;
-    unsigned *bm;
-    int w0;
-    struct Item {
-        int c;
-        V b;
-    };
-    __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
-    __device__ __forceinline__ void apply(const Item &it, V a, int)
-    {
-        const int c = it.c - w0;
-        atomicAdd(acc + c, a * it.b);
-        const unsigned bit = 1u << (c & 31);
-        if (!(bm[c >> 5] & bit)) atomicOr(bm + (c >> 5), bit);
     }
 };
 
@@ -171,7 +152,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
         __syncthreads();
         const FullRow range{brpt};
         NumInsert<T, kHashFib, V> op{bcol, bval, kt, vt};
-        for_each_product<kWave, true, V>(arpt[r], arpt[r + 1], wave, NU, acol, aval, range, op);
+        for_each_product<kWave, true, V, (BS > 256)>(arpt[r], arpt[r + 1], wave, NU, acol, aval, range, op);
         __syncthreads();
         const long long off = crpt[r];
         const int n = min(static_cast<int>(crpt[r + 1] - off), T / 2);  // rows of this bin have n <= T/2
@@ -218,33 +199,127 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
 
 constexpr int kLongBS = 1024;
 
-// Bin 7: rows above 4096 entries.  Dense LDS accumulator + occupancy bitmap
-// per window of W columns; windows are written out in column order, so the
-// output is sorted without a sort.  cursor[] holds two ints per A-nonzero
-// index (span = highest A-nonzero index + 1): windows alternate which half
-// they read and which they write, so waves that share an A chunk (and all
-// compute the same cursor) never read a position another wave has already
-// advanced.
+// Bin 7: rows above 4096 entries, bitmap-rank windows.  The columns are
+// split into windows of WB = 2^18; per window:
+//  1. mark: every product of the window sets its column's bit in an LDS
+//     bitmap (read first, atomic OR only for a clear bit);
+//  2. rank: a block scan of the per-word popcounts gives each word the
+//     number of the row's columns before it, so the position of column c in
+//     the window's sorted output is rk[word] + popcount(lower bits);
+//  3. accumulate: the products are walked again and a*b is added to a
+//     compact LDS value array at that position.  When the window holds more
+//     than VCAP columns, this pass runs over column sub-windows of at most
+//     VCAP columns (cut at the column of rank r0 + VCAP).  With one or two
+//     sub-windows each walk reuses the B ranges step 1 found and skips the
+//     ranks of the other sub-window; with more, each sub-window locates its
+//     own B ranges with cursors, so no product is visited more than twice;
+//  4. write: the window's bits are read out in order with their values.
+// No sort is needed and the value array only holds columns that exist, so a
+// row of 30K entries over 1M columns takes 4 bitmap windows instead of 64
+// dense ones.  Windows with no product are skipped (the next window starts
+// at the smallest column beyond the current one).
+//
+// cursor[] holds four arrays of span ints (span = highest A-nonzero index +
+// 1): two hold the window-start positions of the A-nonzeros' B rows and
+// alternate between windows, two alternate between the sub-windows of
+// step 3.  A walk never writes the array it reads, so waves that share an A
+// chunk (and compute the same positions) cannot see a position another
+// wave has already advanced.
 template <typename V>
-__global__ __launch_bounds__(kLongBS) void k_num_long(const int *__restrict__ arpt, const int *__restrict__ acol,
+struct RankWin {
+    static constexpr int WB = 1 << 18;                          // columns per bitmap window
+    static constexpr int WORDS = WB / 32;                       // bitmap words (32 KB)
+    static constexpr int VCAP = sizeof(V) == 8 ? 8192 : 16384;  // values per accumulate pass (64 KB)
+    static constexpr size_t lds = sizeof(V) * VCAP + sizeof(uint2) * WORDS;
+};
+
+// The window's bitmap words and their ranks are interleaved (x: bits,
+// y: number of set bits in the words before), so the rank of a column is
+// one 64-bit LDS read.
+// Step 1 consumer: set the product's column bit.
+template <typename V>
+struct RankMark {
+    const int *__restrict__ bcol;
+    uint2 *br;
+    int w0;
+    using Item = int;
+    __device__ __forceinline__ int load(int k) const { return bcol[k]; }
+    __device__ __forceinline__ unsigned prep(int col) const { return br[(col - w0) >> 5].x; }
+    __device__ __forceinline__ void commit(int col, unsigned word, V, int)
+    {
+        const int c = col - w0;
+        const unsigned bit = 1u << (c & 31);
+        if (!(word & bit)) atomicOr(&br[c >> 5].x, bit);
+    }
+};
+
+// Step 3 consumer: add a*b at the column's rank relative to r0, for ranks
+// in [r0, r0 + nr) (products of other sub-windows are skipped).
+template <typename V>
+struct RankAcc {
+    const int *__restrict__ bcol;
+    const V *__restrict__ bval;
+    const uint2 *br;
+    V *vals;
+    int w0;
+    int r0;
+    unsigned nr;
+    struct Item {
+        int c;
+        V b;
+    };
+    __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
+    __device__ __forceinline__ uint2 prep(const Item &it) const { return br[(it.c - w0) >> 5]; }
+    __device__ __forceinline__ void commit(const Item &it, const uint2 &e, V a, int)
+    {
+        const int c = it.c - w0;
+        const unsigned at = e.y + __popc(e.x & ((1u << (c & 31)) - 1u)) - static_cast<unsigned>(r0);
+        if (at < nr) atomicAdd(vals + at, a * it.b);
+    }
+};
+
+// B range of an A-nonzero recorded by an earlier walk of the same window:
+// [start[e] (or the row start on the first window), end[e]).
+struct StoredRange {
+    const int *__restrict__ brpt;
+    const int *__restrict__ start;
+    const int *__restrict__ end;
+    bool first;
+    __device__ __forceinline__ void operator()(int e, int k, int &bs, int &len) const
+    {
+        const int lo = first ? brpt[k] : start[e];
+        bs = lo;
+        len = end[e] - lo;
+    }
+};
+
+template <typename V>
+__global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const V *__restrict__ aval, const int *__restrict__ brpt,
                                                       const int *__restrict__ bcol, const V *__restrict__ bval,
                                                       const int *__restrict__ rows, int nrows, int ncols,
                                                       const long long *__restrict__ crpt, int *__restrict__ ccol,
                                                       V *__restrict__ cval, int *__restrict__ cursor, int span)
 {
-    using D = DenseWin<V>;
+    using R = RankWin<V>;
+    constexpr int NU = kLongBS / kWave;
+    constexpr int WPT = R::WORDS / kLongBS;  // bitmap words per thread in the scans
+    static_assert(R::WORDS % kLongBS == 0, "words per thread");
     extern __shared__ unsigned long long smem_u64[];
-    V *acc = reinterpret_cast<V *>(smem_u64);
-    unsigned *bm = reinterpret_cast<unsigned *>(acc + D::W);
+    V *vals = reinterpret_cast<V *>(smem_u64);
+    uint2 *br = reinterpret_cast<uint2 *>(vals + R::VCAP);
     __shared__ int s_scan[kLongBS / kWave + 1];
     __shared__ int s_min[kLongBS / kWave];
-    constexpr int NU = kLongBS / kWave;
+    __shared__ int s_cut;
+    __shared__ FlatStage<kWave, V> s_stage[kLongBS / kWave];
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
+    const int wb0 = tid * WPT;
+    FlatStage<kWave, V> *const stage = s_stage + wave;
+    int *const cs[2] = {cursor, cursor + span};
+    int *const cz[2] = {cursor + 2 * static_cast<long long>(span), cursor + 3 * static_cast<long long>(span)};
 
-    for (int i = tid; i < D::W; i += kLongBS) acc[i] = V(0);
-    for (int i = tid; i < D::WORDS; i += kLongBS) bm[i] = 0u;
+    for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
     __syncthreads();
 
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
@@ -264,49 +339,105 @@ __global__ __launch_bounds__(kLongBS) void k_num_long(const int *__restrict__ ar
         mn = INT_MAX;
         for (int w = 0; w < NU; w++) mn = min(mn, s_min[w]);
         __syncthreads();
-        int w0 = mn == INT_MAX ? ncols : (mn / D::W) * D::W;
+        int w0 = mn == INT_MAX ? ncols : (mn / R::WB) * R::WB;
         bool first = true;
-        int parity = 0;
+        int p = 0;
         while (w0 < ncols) {
-            const int wend = min(ncols, w0 + D::W);
+            const int wend = (ncols - w0 <= R::WB) ? ncols : w0 + R::WB;
+            const bool last = wend >= ncols;
+            // 1. mark
             int nxt = INT_MAX;
-            int *cur_a = cursor + (parity ? span : 0);
-            int *cur_b = cursor + (parity ? 0 : span);
-            const CursorRange range{brpt, bcol, cur_a, cur_b, wend, first, wend >= ncols, &nxt};
-            DenseInsert<V> op{bcol, bval, acc, bm, w0};
-            for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op);
+            {
+                const CursorRange range{brpt, bcol, cs[p], cs[p ^ 1], wend, first, last, &nxt};
+                RankMark<V> op{bcol, br, w0};
+                for_each_product<kWave, false, V>(a_beg, a_end, wave, NU, acol, aval, range, op, stage);
+            }
             nxt = unit_min<kWave>(nxt);
             if (lane_id() == 0) s_min[wave] = nxt;
             __syncthreads();
-            // Read the window out in column order.
-            for (int wb = 0; wb < D::WORDS; wb += kLongBS) {
-                const int word = wb + tid;
-                const unsigned bits = word < D::WORDS ? bm[word] : 0u;
-                int tot;
-                const int pos = block_excl_scan<kLongBS>(__popc(bits), s_scan, tot);
-                unsigned b = bits;
-                int i = 0;
-                while (b) {
-                    const int bit = __ffs(b) - 1;
-                    b &= b - 1;
-                    const int c = word * 32 + bit;
-                    const long long at = out + pos + i;
-                    if (at < row_end) {
-                        ccol[at] = w0 + c;
-                        cval[at] = acc[c];
-                    }
-                    acc[c] = V(0);
-                    i++;
-                }
-                if (word < D::WORDS) bm[word] = 0u;
-                out += tot;
+            // 2. rank
+            int local = 0;
+            for (int i = 0; i < WPT; i++) local += __popc(br[wb0 + i].x);
+            int tot;
+            const int first_rank = block_excl_scan<kLongBS>(local, s_scan, tot);
+            int run = first_rank;
+            for (int i = 0; i < WPT; i++) {
+                br[wb0 + i].y = static_cast<unsigned>(run);
+                run += __popc(br[wb0 + i].x);
             }
             nxt = INT_MAX;
             for (int w = 0; w < NU; w++) nxt = min(nxt, s_min[w]);
             __syncthreads();
+            // 3 + 4, per sub-window [ca, cb) of ranks [r0, r1)
+            int r0 = 0, ca = w0, sub = 0;
+            while (r0 < tot) {
+                const int r1 = min(tot, r0 + R::VCAP);
+                int cb = wend;
+                if (r1 < tot) {
+                    // The thread whose words hold rank r1 finds its column.
+                    if (first_rank <= r1 && r1 < run) {
+                        for (int i = 0; i < WPT; i++) {
+                            const unsigned bits = br[wb0 + i].x;
+                            const int base = static_cast<int>(br[wb0 + i].y);
+                            if (r1 >= base && r1 < base + __popc(bits)) {
+                                unsigned b = bits;
+                                for (int skip = r1 - base; skip > 0; skip--) b &= b - 1u;
+                                s_cut = w0 + (wb0 + i) * 32 + (__ffs(b) - 1);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    cb = s_cut;
+                }
+                for (int i = tid; i < r1 - r0; i += kLongBS) vals[i] = V(0);
+                __syncthreads();
+                RankAcc<V> op{bcol, bval, br, vals, w0, r0, static_cast<unsigned>(r1 - r0)};
+                if (tot <= 2 * R::VCAP) {
+                    // At most two sub-windows: walk the window's products over
+                    // the ranges step 1 recorded and skip the other sub-window's
+                    // ranks, instead of locating a cut in every B row.
+                    const StoredRange range{brpt, cs[p], cs[p ^ 1], first};
+                    for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op, stage);
+                } else {
+                    int unused = INT_MAX;
+                    const int *cin = sub == 0 ? cs[p] : cz[(sub - 1) & 1];
+                    const CursorRange range{brpt,       bcol, cin, cz[sub & 1], cb, first && sub == 0,
+                                            last && cb == wend, &unused};
+                    for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op, stage);
+                }
+                __syncthreads();
+                const int wa = (ca - w0) >> 5;
+                const int we = (cb - w0 + 31) >> 5;
+                for (int w = wa + tid; w < we; w += kLongBS) {
+                    const uint2 ew = br[w];
+                    const unsigned all = ew.x;
+                    unsigned bits = all;
+                    const int lo = ca - w0 - w * 32;
+                    const int hi = cb - w0 - w * 32;
+                    if (lo > 0) bits &= ~((1u << lo) - 1u);
+                    if (hi < 32) bits &= (1u << hi) - 1u;
+                    while (bits) {
+                        const int bit = __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        const int rr = static_cast<int>(ew.y) + __popc(all & ((1u << bit) - 1u));
+                        const long long at = out + rr;
+                        if (at < row_end) {
+                            ccol[at] = w0 + w * 32 + bit;
+                            cval[at] = vals[rr - r0];
+                        }
+                    }
+                }
+                __syncthreads();
+                r0 = r1;
+                ca = cb;
+                sub++;
+            }
+            for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
+            __syncthreads();
+            out += tot;
             first = false;
-            parity ^= 1;
-            w0 = nxt == INT_MAX ? ncols : (nxt / D::W) * D::W;
+            p ^= 1;
+            w0 = nxt == INT_MAX ? ncols : (nxt / R::WB) * R::WB;
         }
     }
 }
@@ -349,8 +480,8 @@ void set_attrs()
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(block_lds<8192, 1024, V>()));
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_block<2048, 256, V>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(block_lds<2048, 256, V>()));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_long<V>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(DenseWin<V>::lds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(RankWin<V>::lds));
 }
 
 }  // namespace
@@ -375,7 +506,7 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows
         case 5: launch_block<2048, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
         case 6: launch_block<8192, 1024, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
         default:
-            k_num_long<V><<<nrows, kLongBS, DenseWin<V>::lds, s>>>(
+            k_num_rank<V><<<nrows, kLongBS, RankWin<V>::lds, s>>>(
                 A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
                 B.N, crpt, ccol, cval, cursor, span);
             break;

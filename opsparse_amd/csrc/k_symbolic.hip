@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
         r = rows ? rows[ridx] : ridx;
         SymInsert<T, H> op{bcol, tab + slot * T};
         const FullRow range{brpt};
-        for_each_product<U, false, double>(arpt[r], arpt[r + 1], unit, NU, acol, static_cast<const double *>(nullptr),
+        for_each_product<U, false, double, true>(arpt[r], arpt[r + 1], unit, NU, acol, static_cast<const double *>(nullptr),
                                            range, op);
         cnt = op.cnt;
     }
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
             __syncthreads();
             const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols};
             BitmapInsert op{bcol, bm, w0};
-            for_each_product<kWave, false, double>(a_beg, a_end, wave, NW, acol, static_cast<const double *>(nullptr),
+            for_each_product<kWave, false, double, true>(a_beg, a_end, wave, NW, acol, static_cast<const double *>(nullptr),
                                                    range, op);
             cnt += op.cnt;
             __syncthreads();

@@ -98,7 +98,7 @@ struct PlanCore {
     long long nnz = 0;
     long long *crpt64 = nullptr;  // M+1, device
     int *bins = nullptr;          // numeric bin lists (nullptr: identity over one bin)
-    int *cursor = nullptr;        // two ints per A-nonzero index, long-row bin only
+    int *cursor = nullptr;        // four ints per A-nonzero index, long-row bin only
     int span = 0;                 // highest A-nonzero index + 1
     int num_size[kNumBins] = {};
     int num_off[kNumBins] = {};
@@ -407,7 +407,7 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
     }
     if (pc.num_size[kNumBins - 1] > 0) {
         pc.span = static_cast<int>(a_index_span(A, s));
-        ws->cursor.ensure(2 * static_cast<size_t>(pc.span) + 2);
+        ws->cursor.ensure(4 * static_cast<size_t>(pc.span) + 4);
         pc.cursor = ws->cursor.p;
     }
     if (t) t->allocate = now_s() - t0;
@@ -716,7 +716,7 @@ int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_p
         symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr);
         if (p->core.num_size[kNumBins - 1] > 0) {
             p->core.span = static_cast<int>(a_index_span(A, s));
-            p->cursor.ensure(2 * static_cast<size_t>(p->core.span) + 2);
+            p->cursor.ensure(4 * static_cast<size_t>(p->core.span) + 4);
             p->core.cursor = p->cursor.p;
         }
         CK(hipStreamSynchronize(s));

@@ -94,7 +94,7 @@ def test_every_bin_rmat(gpu):
 
 def test_long_rows_windows(gpu):
     """Rows with > 16384 products (symbolic long path) and > 4096 entries
-    (numeric dense windows over several 16384-column windows)."""
+    (numeric bitmap-rank windows)."""
     rng = np.random.default_rng(3)
     N = 50000
     A = random_csr(40, N, np.r_[rng.integers(150, 260, 20), rng.integers(1, 4, 20)], rng)
@@ -178,3 +178,16 @@ def test_output_cache_reuse_and_limit(gpu):
     assert mem_cache_stats()["cached_bytes"] == 0
     set_mem_cache_limit(-1)
     mem_cache_release()
+
+
+def test_long_rows_many_rank_subwindows(gpu):
+    """Rows of ~90K entries inside one 2^18-column bitmap window: the
+    accumulate pass runs over many value sub-windows (8192 FP64 / 16384 FP32
+    entries each), cut at the column of each rank boundary."""
+    rng = np.random.default_rng(12)
+    N = 200000
+    A = random_csr(6, 1000, np.r_[rng.integers(350, 450, 4), rng.integers(1, 3, 2)], rng)
+    B = random_csr(1000, N, rng.integers(250, 350, 1000), rng)
+    run(A, B)
+    run(A, B, np.float32)
+    run(A, B, wide=True)
