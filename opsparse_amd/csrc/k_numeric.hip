@@ -213,7 +213,8 @@ constexpr int kLongBS = 1024;
 //     sub-windows each walk reuses the B ranges step 1 found and skips the
 //     ranks of the other sub-window; with more, each sub-window locates its
 //     own B ranges with cursors, so no product is visited more than twice;
-//  4. write: the window's bits are read out in order with their values.
+//  4. write: values (already in rank order) and columns (placed by rank in
+//     LDS) are copied out contiguously.
 // No sort is needed and the value array only holds columns that exist, so a
 // row of 30K entries over 1M columns takes 4 bitmap windows instead of 64
 // dense ones.  Windows with no product are skipped (the next window starts
@@ -406,6 +407,16 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op, stage);
                 }
                 __syncthreads();
+                // 4. write: the values are already in rank order, so they
+                // go out as one contiguous copy; the columns are first
+                // placed by rank in LDS (over the values just written) and
+                // then copied the same way, so every global store is
+                // coalesced.
+                const int nsub = r1 - r0;
+                const long long lim = min(static_cast<long long>(nsub), row_end - (out + r0));
+                for (int i = tid; i < lim; i += kLongBS) cval[out + r0 + i] = vals[i];
+                __syncthreads();
+                int *scol = reinterpret_cast<int *>(vals);
                 const int wa = (ca - w0) >> 5;
                 const int we = (cb - w0 + 31) >> 5;
                 for (int w = wa + tid; w < we; w += kLongBS) {
@@ -419,14 +430,11 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     while (bits) {
                         const int bit = __ffs(bits) - 1;
                         bits &= bits - 1u;
-                        const int rr = static_cast<int>(ew.y) + __popc(all & ((1u << bit) - 1u));
-                        const long long at = out + rr;
-                        if (at < row_end) {
-                            ccol[at] = w0 + w * 32 + bit;
-                            cval[at] = vals[rr - r0];
-                        }
+                        scol[static_cast<int>(ew.y) + __popc(all & ((1u << bit) - 1u)) - r0] = w0 + w * 32 + bit;
                     }
                 }
+                __syncthreads();
+                for (int i = tid; i < lim; i += kLongBS) ccol[out + r0 + i] = scol[i];
                 __syncthreads();
                 r0 = r1;
                 ca = cb;
