@@ -122,9 +122,57 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     }
 }
 
-// Bins 5-6: one row per block of BS threads; table of T slots in dynamic
-// LDS, then compaction of the keys and an LDS bitonic sort of the keys; the
-// values are read back from the table by key.
+// The window's bitmap words and their ranks are interleaved (x: bits,
+// y: number of set bits in the words before), so the rank of a column is
+// one 64-bit LDS read.
+// Step 1 consumer: set the product's column bit.
+template <typename V>
+struct RankMark {
+    const int *__restrict__ bcol;
+    uint2 *br;
+    int w0;
+    using Item = int;
+    __device__ __forceinline__ int load(int k) const { return bcol[k]; }
+    __device__ __forceinline__ unsigned prep(int col) const { return br[(col - w0) >> 5].x; }
+    __device__ __forceinline__ void commit(int col, unsigned word, V, int)
+    {
+        const int c = col - w0;
+        const unsigned bit = 1u << (c & 31);
+        if (!(word & bit)) atomicOr(&br[c >> 5].x, bit);
+    }
+};
+
+// Step 3 consumer: add a*b at the column's rank relative to r0, for ranks
+// in [r0, r0 + nr) (products of other sub-windows are skipped).
+template <typename V>
+struct RankAcc {
+    const int *__restrict__ bcol;
+    const V *__restrict__ bval;
+    const uint2 *br;
+    V *vals;
+    int w0;
+    int r0;
+    unsigned nr;
+    struct Item {
+        int c;
+        V b;
+    };
+    __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
+    __device__ __forceinline__ uint2 prep(const Item &it) const { return br[(it.c - w0) >> 5]; }
+    __device__ __forceinline__ void commit(const Item &it, const uint2 &e, V a, int)
+    {
+        const int c = it.c - w0;
+        const unsigned at = e.y + __popc(e.x & ((1u << (c & 31)) - 1u)) - static_cast<unsigned>(r0);
+        if (at < nr) atomicAdd(vals + at, a * it.b);
+    }
+};
+
+// Bins 5-6: one row per block of BS threads, at most T/2 entries per row.
+// Rows whose columns span fewer than 32*T columns take the bitmap-rank path
+// of the long-row kernel in one window (a bitmap word per 32 columns from
+// the row's smallest column; no hash, no sort).  Other rows use a table of
+// T slots in dynamic LDS, then compaction of the keys and an LDS bitonic
+// sort of the keys; the values are read back from the table by key.
 template <int T, int BS, typename V>
 __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                   const V *__restrict__ aval, const int *__restrict__ brpt,
@@ -134,28 +182,109 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
                                                   V *__restrict__ cval)
 {
     extern __shared__ unsigned long long smem_u64[];
-    V *vt = reinterpret_cast<V *>(smem_u64);   // T values
-    int *kt = reinterpret_cast<int *>(vt + T);  // T keys
-    int *sk = kt + T;                           // T/2 sorted keys
+    // Hash path: T values | T keys | T/2 sorted keys.
+    V *vt = reinterpret_cast<V *>(smem_u64);
+    int *kt = reinterpret_cast<int *>(vt + T);
+    int *sk = kt + T;
+    // Rank path over the same memory: T bitmap words with ranks | T/2 values.
+    uint2 *br = reinterpret_cast<uint2 *>(smem_u64);
+    V *rv = reinterpret_cast<V *>(br + T);
+    static_assert(sizeof(uint2) * T + sizeof(V) * (T / 2) <= sizeof(V) * T + sizeof(int) * (T + T / 2),
+                  "rank path must fit in the hash path's LDS");
     __shared__ int s_pos;
+    __shared__ int s_scan[BS / kWave + 1];
+    __shared__ int s_mm[2 * (BS / kWave)];
     constexpr int NU = BS / kWave;
+    constexpr int WPT = T / BS;  // bitmap words per thread in the rank scan
+    static_assert(T % BS == 0, "words per thread");
+    constexpr bool SL = BS > 256;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
+    const FullRow range{brpt};
 
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
         const int r = rows ? rows[ridx] : ridx;
+        const int a_beg = arpt[r], a_end = arpt[r + 1];
+        const long long off = crpt[r];
+        const int n = min(static_cast<int>(crpt[r + 1] - off), T / 2);  // rows of this bin have n <= T/2
+        // Column span of the row: first and last columns of its B rows.
+        int mn = INT_MAX, mxc = -1;
+        for (int e = a_beg + tid; e < a_end; e += BS) {
+            const int k = acol[e];
+            const int b0 = brpt[k], b1 = brpt[k + 1];
+            if (b0 < b1) {
+                mn = min(mn, bcol[b0]);
+                mxc = max(mxc, bcol[b1 - 1]);
+            }
+        }
+        mn = unit_min<kWave>(mn);
+        mxc = unit_max<kWave>(mxc);
+        if (lane_id() == 0) {
+            s_mm[wave] = mn;
+            s_mm[NU + wave] = mxc;
+        }
+        __syncthreads();
+        mn = INT_MAX;
+        mxc = -1;
+        for (int w = 0; w < NU; w++) {
+            mn = min(mn, s_mm[w]);
+            mxc = max(mxc, s_mm[NU + w]);
+        }
+        if (mxc >= 0 && mxc - mn < 32 * T) {
+            const int nw = ((mxc - mn) >> 5) + 1;
+            for (int i = tid; i < nw; i += BS) br[i] = make_uint2(0u, 0u);
+            __syncthreads();
+            {
+                RankMark<V> op{bcol, br, mn};
+                for_each_product<kWave, false, V, SL>(a_beg, a_end, wave, NU, acol, aval, range, op);
+            }
+            __syncthreads();
+            const int wb0 = tid * WPT;
+            int local = 0;
+            for (int i = 0; i < WPT; i++)
+                if (wb0 + i < nw) local += __popc(br[wb0 + i].x);
+            int tot;
+            int run = block_excl_scan<BS>(local, s_scan, tot);
+            for (int i = 0; i < WPT; i++) {
+                if (wb0 + i < nw) {
+                    br[wb0 + i].y = static_cast<unsigned>(run);
+                    run += __popc(br[wb0 + i].x);
+                }
+            }
+            for (int i = tid; i < n; i += BS) rv[i] = V(0);
+            __syncthreads();
+            {
+                RankAcc<V> op{bcol, bval, br, rv, mn, 0, static_cast<unsigned>(n)};
+                for_each_product<kWave, true, V, SL>(a_beg, a_end, wave, NU, acol, aval, range, op);
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += BS) cval[off + i] = rv[i];
+            __syncthreads();
+            int *scol = reinterpret_cast<int *>(rv);
+            for (int w = tid; w < nw; w += BS) {
+                const uint2 ew = br[w];
+                unsigned bits = ew.x;
+                while (bits) {
+                    const int bit = __ffs(bits) - 1;
+                    bits &= bits - 1u;
+                    const int rr = static_cast<int>(ew.y) + __popc(ew.x & ((1u << bit) - 1u));
+                    if (rr < n) scol[rr] = mn + w * 32 + bit;
+                }
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += BS) ccol[off + i] = scol[i];
+            __syncthreads();
+            continue;
+        }
         for (int i = tid; i < T; i += BS) {
             kt[i] = -1;
             vt[i] = V(0);
         }
         if (tid == 0) s_pos = 0;
         __syncthreads();
-        const FullRow range{brpt};
         NumInsert<T, kHashFib, V> op{bcol, bval, kt, vt};
-        for_each_product<kWave, true, V, (BS > 256)>(arpt[r], arpt[r + 1], wave, NU, acol, aval, range, op);
+        for_each_product<kWave, true, V, SL>(a_beg, a_end, wave, NU, acol, aval, range, op);
         __syncthreads();
-        const long long off = crpt[r];
-        const int n = min(static_cast<int>(crpt[r + 1] - off), T / 2);  // rows of this bin have n <= T/2
         for (int i = tid; i < T; i += BS) {
             const int key = kt[i];
             const bool valid = key != -1;
@@ -232,51 +361,6 @@ struct RankWin {
     static constexpr int WORDS = WB / 32;                       // bitmap words (32 KB)
     static constexpr int VCAP = sizeof(V) == 8 ? 8192 : 16384;  // values per accumulate pass (64 KB)
     static constexpr size_t lds = sizeof(V) * VCAP + sizeof(uint2) * WORDS;
-};
-
-// The window's bitmap words and their ranks are interleaved (x: bits,
-// y: number of set bits in the words before), so the rank of a column is
-// one 64-bit LDS read.
-// Step 1 consumer: set the product's column bit.
-template <typename V>
-struct RankMark {
-    const int *__restrict__ bcol;
-    uint2 *br;
-    int w0;
-    using Item = int;
-    __device__ __forceinline__ int load(int k) const { return bcol[k]; }
-    __device__ __forceinline__ unsigned prep(int col) const { return br[(col - w0) >> 5].x; }
-    __device__ __forceinline__ void commit(int col, unsigned word, V, int)
-    {
-        const int c = col - w0;
-        const unsigned bit = 1u << (c & 31);
-        if (!(word & bit)) atomicOr(&br[c >> 5].x, bit);
-    }
-};
-
-// Step 3 consumer: add a*b at the column's rank relative to r0, for ranks
-// in [r0, r0 + nr) (products of other sub-windows are skipped).
-template <typename V>
-struct RankAcc {
-    const int *__restrict__ bcol;
-    const V *__restrict__ bval;
-    const uint2 *br;
-    V *vals;
-    int w0;
-    int r0;
-    unsigned nr;
-    struct Item {
-        int c;
-        V b;
-    };
-    __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
-    __device__ __forceinline__ uint2 prep(const Item &it) const { return br[(it.c - w0) >> 5]; }
-    __device__ __forceinline__ void commit(const Item &it, const uint2 &e, V a, int)
-    {
-        const int c = it.c - w0;
-        const unsigned at = e.y + __popc(e.x & ((1u << (c & 31)) - 1u)) - static_cast<unsigned>(r0);
-        if (at < nr) atomicAdd(vals + at, a * it.b);
-    }
 };
 
 // B range of an A-nonzero recorded by an earlier walk of the same window:

@@ -191,3 +191,16 @@ def test_long_rows_many_rank_subwindows(gpu):
     run(A, B)
     run(A, B, np.float32)
     run(A, B, wide=True)
+
+
+def test_block_bins_rank_and_hash_paths(gpu):
+    """Rows of 256..4096 entries: a narrow column span takes the bitmap-rank
+    path of the block bins, a span over 2^21 columns the hash + sort path."""
+    rng = np.random.default_rng(13)
+    for N in (40000, (1 << 21) + 5):
+        K = 3000
+        A = random_csr(96, K, np.r_[rng.integers(2, 7, 48), rng.integers(15, 26, 48)], rng)
+        B = random_csr(K, N, rng.integers(80, 150, K), rng)
+        log(f"block bins N={N}")
+        run(A, B)
+        run(A, B, np.float32)
