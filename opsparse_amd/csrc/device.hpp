@@ -461,13 +461,6 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
     }
 }
 
-template <typename V>
-struct DenseWin {
-    static constexpr int W = sizeof(V) == 8 ? 16384 : 32768;  // columns per window: 128 KB of values
-    static constexpr int WORDS = W / 32;
-    static constexpr size_t lds = sizeof(V) * W + sizeof(unsigned) * WORDS;
-};
-
 // Cursor-carrying B range for window [w0, wend): lo is the first entry of
 // the B row not consumed by earlier windows; hi the first entry >= wend,
 // found by galloping from lo.  Tracks the smallest column beyond the window

@@ -8,11 +8,12 @@
 //  * rows up to 4096 entries: row i of C (its columns and dC values) is
 //    loaded into the same LDS open-addressing table the forward numeric
 //    kernels use, keyed by column;
-//  * longer rows: the row is split into dense column windows; each window's
-//    dC values are staged in LDS indexed by column, and B rows are consumed
-//    window by window through per-A-nonzero cursors (as in the forward's
-//    long-row kernel).  Windows without entries in row i of C hold no
-//    products and are skipped by reading the next column of the C row.
+//  * longer rows: bitmap-rank windows built from row i of C (k_bwd_rank):
+//    dC is staged in LDS in C-row order and looked up by the column's rank;
+//    B rows are consumed window by window through per-A-nonzero cursors (as
+//    in the forward's long-row kernel).  Windows without entries in row i of
+//    C hold no products and are skipped by reading the next column of the C
+//    row.
 // dA is reduced per A-nonzero in LDS, then added to da once per chunk; dB is
 // scattered with global atomic adds.
 #include <algorithm>
@@ -79,30 +80,6 @@ struct BwdHash {
     __device__ __forceinline__ void chunk_end(int e, bool valid, bool shared) { red.flush(e, valid, shared); }
 };
 
-// Consumer for one dense window [w0, w0 + W) of a long C row (g: dC by column - w0).
-template <int U, typename V>
-struct BwdWin {
-    const int *__restrict__ bcol;
-    const V *__restrict__ bval;
-    const V *g;
-    int w0;
-    V *__restrict__ db;
-    DaReduce<U, V> red;
-    struct Item {
-        int k;
-        int c;
-        V b;
-    };
-    __device__ __forceinline__ Item load(int k) const { return Item{k, bcol[k], bval[k]}; }
-    __device__ __forceinline__ V prep(const Item &it) const { return g[it.c - w0]; }
-    __device__ __forceinline__ void commit(const Item &it, const V &gv, V a, int q)
-    {
-        global_add(db + it.k, a * gv);
-        red.add(q, gv * it.b);
-    }
-    __device__ __forceinline__ void chunk_end(int e, bool valid, bool shared) { red.flush(e, valid, shared); }
-};
-
 // G lanes per row (G <= 64: BS/G rows per block; G > 64: one row per block,
 // G/64 waves share the table).  Dynamic LDS: vals[RPB*T] | sda[BS] | keys[RPB*T].
 template <int G, int T, int BS, typename V>
@@ -164,30 +141,77 @@ __global__ void k_zero_vals_f32(float *p, long long n)
         p[i] = 0.f;
 }
 
-// Rows of C above 4096 entries: dense dC windows of W columns (dynamic LDS
-// of W values).  cursor[] holds two ints per A-nonzero index (span = highest
-// A-nonzero index + 1); windows alternate which half they read and which
-// they write, so waves that share an A chunk never read a position another
-// wave has already advanced.
+// Rows of C above 4096 entries: bitmap-rank windows, as in the forward's
+// long-row kernel, with the bitmap built from row i of C itself.  Per
+// window of WB = 2^18 columns: the columns of the C row inside it set their
+// bits, a block scan gives every word its rank (the rank of column c is
+// then its position among the row's entries of the window), and for each
+// sub-window of up to VCAP entries the dC values are staged in LDS in C-row
+// order (a contiguous copy).  The products of the sub-window, found through
+// per-A-nonzero cursors, look their dC up by rank.  cursor[] holds two ints
+// per A-nonzero index (span = highest A-nonzero index + 1); sub-windows
+// alternate which half they read and which they write, so waves that share
+// an A chunk never read a position another wave has already advanced.
 template <typename V>
-__global__ __launch_bounds__(kLongBS) void k_bwd_win(const int *__restrict__ arpt, const int *__restrict__ acol,
-                                                     const V *__restrict__ aval, const int *__restrict__ brpt,
-                                                     const int *__restrict__ bcol, const V *__restrict__ bval,
-                                                     const int *__restrict__ rows, int nrows, int ncols,
-                                                     const long long *__restrict__ crpt,
-                                                     const int *__restrict__ ccol, const V *__restrict__ dc,
-                                                     V *__restrict__ da, V *__restrict__ db, int *__restrict__ cursor,
-                                                     int span)
+struct BwdRankWin {
+    static constexpr int WB = 1 << 18;
+    static constexpr int WORDS = WB / 32;
+    static constexpr int VCAP = sizeof(V) == 8 ? 8192 : 16384;
+    static constexpr size_t lds = sizeof(uint2) * WORDS + sizeof(V) * VCAP;
+};
+
+// Consumer for one sub-window: dC of the product's column at rank - r0.
+template <int U, typename V>
+struct BwdRank {
+    const int *__restrict__ bcol;
+    const V *__restrict__ bval;
+    const uint2 *br;
+    const V *gv;
+    int w0;
+    int r0;
+    V *__restrict__ db;
+    DaReduce<U, V> red;
+    struct Item {
+        int k;
+        int c;
+        V b;
+    };
+    __device__ __forceinline__ Item load(int k) const { return Item{k, bcol[k], bval[k]}; }
+    __device__ __forceinline__ uint2 prep(const Item &it) const { return br[(it.c - w0) >> 5]; }
+    __device__ __forceinline__ void commit(const Item &it, const uint2 &e, V a, int q)
+    {
+        const int c = it.c - w0;
+        const V g = gv[static_cast<int>(e.y) + __popc(e.x & ((1u << (c & 31)) - 1u)) - r0];
+        global_add(db + it.k, a * g);
+        red.add(q, g * it.b);
+    }
+    __device__ __forceinline__ void chunk_end(int e, bool valid, bool shared) { red.flush(e, valid, shared); }
+};
+
+template <typename V>
+__global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                      const V *__restrict__ aval, const int *__restrict__ brpt,
+                                                      const int *__restrict__ bcol, const V *__restrict__ bval,
+                                                      const int *__restrict__ rows, int nrows, int ncols,
+                                                      const long long *__restrict__ crpt,
+                                                      const int *__restrict__ ccol, const V *__restrict__ dc,
+                                                      V *__restrict__ da, V *__restrict__ db, int *__restrict__ cursor,
+                                                      int span)
 {
-    using D = DenseWin<V>;
+    using R = BwdRankWin<V>;
+    constexpr int NU = kLongBS / kWave;
+    constexpr int WPT = R::WORDS / kLongBS;
     extern __shared__ unsigned long long smem_u64[];
-    V *g = reinterpret_cast<V *>(smem_u64);
+    uint2 *br = reinterpret_cast<uint2 *>(smem_u64);
+    V *gv = reinterpret_cast<V *>(br + R::WORDS);
     __shared__ V sda[kLongBS];
     __shared__ int s_cnt[kLongBS / kWave];
-    constexpr int NU = kLongBS / kWave;
+    __shared__ int s_scan[kLongBS / kWave + 1];
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
+    const int wb0 = tid * WPT;
     sda[tid] = V(0);
+    for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
     __syncthreads();
 
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
@@ -198,18 +222,16 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_win(const int *__restrict__ arp
         bool first = true;
         int parity = 0;
         while (cpos < c1) {
-            // Window of the next column of row i of C; every product of this
-            // window has its column among the staged entries.
-            const int w0 = (ccol[cpos] / D::W) * D::W;
-            const int wend = (ncols - w0 <= D::W) ? ncols : w0 + D::W;
-            // The row's entries below wend are a prefix of [cpos, c1) of at
-            // most W entries (distinct columns of the window).
-            const long long lim = min(c1, cpos + static_cast<long long>(D::W));
+            // Window of the next column of row i of C: its entries below
+            // wend are a prefix of [cpos, c1) of at most WB entries.
+            const int w0 = (ccol[cpos] / R::WB) * R::WB;
+            const int wend = (ncols - w0 <= R::WB) ? ncols : w0 + R::WB;
+            const long long lim = min(c1, cpos + static_cast<long long>(R::WB));
             int cnt = 0;
             for (long long j = cpos + tid; j < lim; j += kLongBS) {
                 const int c = ccol[j];
                 if (c < wend) {
-                    g[c - w0] = dc[j];
+                    atomicOr(&br[(c - w0) >> 5].x, 1u << ((c - w0) & 31));
                     cnt++;
                 }
             }
@@ -218,16 +240,34 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_win(const int *__restrict__ arp
             __syncthreads();
             int tot = 0;
             for (int w = 0; w < NU; w++) tot += s_cnt[w];
-            int nxt = INT_MAX;
-            const CursorRange range{brpt, bcol, cursor + (parity ? span : 0), cursor + (parity ? 0 : span), wend,
-                                    first, wend >= ncols, &nxt};
-            BwdWin<kWave, V> op{bcol, bval, g, w0, db, DaReduce<kWave, V>{sda + wave * kWave, da}};
-            for_each_product<kWave, true, V, true>(a_beg, a_end, wave, NU, acol, aval, range, op);
-            // Every read of g and s_cnt of this window is done before the next one stages.
+            int local = 0;
+            for (int i = 0; i < WPT; i++) local += __popc(br[wb0 + i].x);
+            int t_all;
+            int run = block_excl_scan<kLongBS>(local, s_scan, t_all);
+            for (int i = 0; i < WPT; i++) {
+                br[wb0 + i].y = static_cast<unsigned>(run);
+                run += __popc(br[wb0 + i].x);
+            }
+            __syncthreads();
+            // Sub-windows of up to VCAP entries [s0, s1) of the C row.
+            for (int r0 = 0; r0 < tot; r0 += R::VCAP) {
+                const int r1 = min(tot, r0 + R::VCAP);
+                const int cb = r1 < tot ? ccol[cpos + r1] : wend;
+                for (int i = tid; i < r1 - r0; i += kLongBS) gv[i] = dc[cpos + r0 + i];
+                __syncthreads();
+                int nxt = INT_MAX;
+                const CursorRange range{brpt, bcol, cursor + (parity ? span : 0), cursor + (parity ? 0 : span), cb,
+                                        first, cb >= ncols, &nxt};
+                BwdRank<kWave, V> op{bcol, bval, br, gv, w0, r0, db, DaReduce<kWave, V>{sda + wave * kWave, da}};
+                for_each_product<kWave, true, V, true>(a_beg, a_end, wave, NU, acol, aval, range, op);
+                // Every read of gv (and of s_cnt) is done before the next stage.
+                __syncthreads();
+                first = false;
+                parity ^= 1;
+            }
+            for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
             __syncthreads();
             cpos += tot;
-            first = false;
-            parity ^= 1;
         }
     }
 }
@@ -266,8 +306,8 @@ void set_attrs()
     set_attr<64, 512, 256, V>();
     set_attr<256, 2048, 256, V>();
     set_attr<1024, 8192, 1024, V>();
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_bwd_win<V>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(sizeof(V) * DenseWin<V>::W));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_bwd_rank<V>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(BwdRankWin<V>::lds));
 }
 
 }  // namespace
@@ -292,7 +332,7 @@ void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *row
         case 5: launch_hash<256, 2048, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
         case 6: launch_hash<1024, 8192, 1024, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
         default:
-            k_bwd_win<V><<<nrows, kLongBS, sizeof(V) * DenseWin<V>::W, s>>>(
+            k_bwd_rank<V><<<nrows, kLongBS, BwdRankWin<V>::lds, s>>>(
                 A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
                 B.N, crpt, ccol, dc, da, db, cursor, span);
             break;

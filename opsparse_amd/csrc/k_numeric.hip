@@ -17,6 +17,8 @@
 // columns per 2^18-column window gives every column its rank (its position
 // in the sorted output), values accumulate at that rank in LDS, and the
 // window is written out in order (k_num_rank).
+#include <algorithm>
+
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -179,9 +181,12 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
                                                   const int *__restrict__ bcol, const V *__restrict__ bval,
                                                   const int *__restrict__ rows, int nrows,
                                                   const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                  V *__restrict__ cval)
+                                                  V *__restrict__ cval, const int *__restrict__ dcount)
 {
     extern __shared__ unsigned long long smem_u64[];
+    // dcount (optional): the number of rows, read on the device (rows that
+    // a previous kernel of the same stream appended to a list).
+    if (dcount) nrows = *dcount;
     // Hash path: T values | T keys | T/2 sorted keys.
     V *vt = reinterpret_cast<V *>(smem_u64);
     int *kt = reinterpret_cast<int *>(vt + T);
@@ -324,6 +329,104 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
         }
         __syncthreads();
     }
+}
+
+// Bin 5 by wave: one row per wave, the bitmap-rank path of the block bins
+// without block barriers.  Each wave has W bitmap words (a column span of
+// up to 32*W from the row's smallest column) and NMAX values in LDS.  Rows
+// with a wider span are appended to spill (spill[0]: count, then row ids)
+// for the block kernel, which runs next on the same stream.
+template <int NMAX, int W, typename V>
+struct WaveRankLds {
+    uint2 br[W];
+    V vals[NMAX];
+};
+
+template <int NMAX, int W, int WPB, typename V>
+__global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict__ arpt, const int *__restrict__ acol,
+                                                           const V *__restrict__ aval, const int *__restrict__ brpt,
+                                                           const int *__restrict__ bcol, const V *__restrict__ bval,
+                                                           const int *__restrict__ rows, int nrows,
+                                                           const long long *__restrict__ crpt, int *__restrict__ ccol,
+                                                           V *__restrict__ cval, int *__restrict__ spill)
+{
+    constexpr int WPL = W / kWave;  // bitmap words per lane in the rank scan
+    static_assert(W % kWave == 0 && NMAX * sizeof(V) >= NMAX * sizeof(int), "layout");
+    __shared__ WaveRankLds<NMAX, W, V> s_lds[WPB];
+    const int wave = threadIdx.x / kWave;
+    const int lane = lane_id();
+    WaveRankLds<NMAX, W, V> &L = s_lds[wave];
+    const int ridx = xcd_tile() * WPB + wave;
+    if (ridx >= nrows) return;  // whole wave: nothing below synchronises the block
+    const int r = rows ? rows[ridx] : ridx;
+    const int a_beg = arpt[r], a_end = arpt[r + 1];
+    int mn = INT_MAX, mx = -1;
+    for (int e = a_beg + lane; e < a_end; e += kWave) {
+        const int k = acol[e];
+        const int b0 = brpt[k], b1 = brpt[k + 1];
+        if (b0 < b1) {
+            mn = min(mn, bcol[b0]);
+            mx = max(mx, bcol[b1 - 1]);
+        }
+    }
+    mn = unit_min<kWave>(mn);
+    mx = unit_max<kWave>(mx);
+    if (mx < 0) return;  // no products: the row is empty
+    if (mx - mn >= 32 * W) {
+        if (lane == 0) spill[1 + atomicAdd(spill, 1)] = r;
+        return;
+    }
+    const int nw = ((mx - mn) >> 5) + 1;
+    for (int i = lane; i < nw; i += kWave) L.br[i] = make_uint2(0u, 0u);
+    wave_lds_sync();
+    const FullRow range{brpt};
+    {
+        RankMark<V> op{bcol, L.br, mn};
+        for_each_product<kWave, false, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
+    }
+    wave_lds_sync();
+    const int wb0 = lane * WPL;
+    int local = 0;
+#pragma unroll
+    for (int i = 0; i < WPL; i++)
+        if (wb0 + i < nw) local += __popc(L.br[wb0 + i].x);
+    int run = unit_incl_scan<kWave>(local) - local;
+#pragma unroll
+    for (int i = 0; i < WPL; i++) {
+        if (wb0 + i < nw) {
+            L.br[wb0 + i].y = static_cast<unsigned>(run);
+            run += __popc(L.br[wb0 + i].x);
+        }
+    }
+    const long long off = crpt[r];
+    const int n = min(static_cast<int>(crpt[r + 1] - off), NMAX);
+    for (int i = lane; i < n; i += kWave) L.vals[i] = V(0);
+    wave_lds_sync();
+    {
+        RankAcc<V> op{bcol, bval, L.br, L.vals, mn, 0, static_cast<unsigned>(n)};
+        for_each_product<kWave, true, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
+    }
+    wave_lds_sync();
+    for (int i = lane; i < n; i += kWave) cval[off + i] = L.vals[i];
+    wave_lds_sync();
+    int *scol = reinterpret_cast<int *>(L.vals);
+    for (int w = lane; w < nw; w += kWave) {
+        const uint2 ew = L.br[w];
+        unsigned bits = ew.x;
+        while (bits) {
+            const int bit = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            const int rr = static_cast<int>(ew.y) + __popc(ew.x & ((1u << bit) - 1u));
+            if (rr < n) scol[rr] = mn + w * 32 + bit;
+        }
+    }
+    wave_lds_sync();
+    for (int i = lane; i < n; i += kWave) ccol[off + i] = scol[i];
+}
+
+__global__ void k_zero_count(int *p)
+{
+    if (threadIdx.x == 0) p[0] = 0;
 }
 
 constexpr int kLongBS = 1024;
@@ -558,11 +661,11 @@ constexpr size_t block_lds()
 
 template <int T, int BS, typename V>
 void launch_block(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt, int *ccol,
-                  V *cval, hipStream_t s)
+                  V *cval, hipStream_t s, const int *dcount = nullptr, int grid = 0)
 {
-    k_num_block<T, BS, V><<<nrows, BS, block_lds<T, BS, V>(), s>>>(A.rpt, A.col, static_cast<const V *>(A.val), B.rpt,
-                                                                   B.col, static_cast<const V *>(B.val), rows, nrows,
-                                                                   crpt, ccol, cval);
+    k_num_block<T, BS, V><<<grid > 0 ? grid : nrows, BS, block_lds<T, BS, V>(), s>>>(
+        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows, crpt, ccol,
+        cval, dcount);
 }
 
 template <typename V>
@@ -586,7 +689,7 @@ void numeric_init_attributes()
 
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, int *cursor, int span, int hash, hipStream_t s)
+                        int *ccol, V *cval, int *cursor, int span, int *spill, int hash, hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
@@ -595,7 +698,19 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows
         case 2: launch_small<64, 128, 64, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
         case 3: launch_small<64, 256, 128, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
         case 4: launch_small<64, 512, 256, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
-        case 5: launch_block<2048, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case kSpillBin:
+            if (spill) {
+                // Wave per row; rows with a wide column span go to the block kernel.
+                constexpr int WPB = 4;
+                k_zero_count<<<1, kWave, 0, s>>>(spill);
+                k_num_wrank<1024, 512, WPB, V><<<xcd_grid((nrows + WPB - 1) / WPB), WPB * kWave, 0, s>>>(
+                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows,
+                    nrows, crpt, ccol, cval, spill);
+                launch_block<2048, 256, V>(A, B, spill + 1, nrows, crpt, ccol, cval, s, spill, std::min(nrows, 4096));
+            } else {
+                launch_block<2048, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s);
+            }
+            break;
         case 6: launch_block<8192, 1024, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
         default:
             k_num_rank<V><<<nrows, kLongBS, RankWin<V>::lds, s>>>(
@@ -606,8 +721,8 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows
 }
 
 template void launch_numeric_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
-                                         double *, int *, int, int, hipStream_t);
+                                         double *, int *, int, int *, int, hipStream_t);
 template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
-                                        float *, int *, int, int, hipStream_t);
+                                        float *, int *, int, int *, int, hipStream_t);
 
 }  // namespace opsamd

@@ -106,9 +106,13 @@ void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *row
 size_t symbolic_long_lds();
 
 // --- numeric (k_numeric.hip) ---
+// Numeric bin run one row per wave first (rows too wide for its bitmap are
+// passed on to the block kernel through `spill`).
+constexpr int kSpillBin = 5;
+// spill: scratch of (rows of bin kSpillBin + 1) ints, or nullptr (block kernel only).
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, int *cursor, int span, int hash, hipStream_t s);
+                        int *ccol, V *cval, int *cursor, int span, int *spill, int hash, hipStream_t s);
 // One-time kernel attribute setup (dynamic LDS limits).
 void numeric_init_attributes();
 void symbolic_init_attributes();

@@ -85,7 +85,7 @@ struct ops_workspace {
     hipEvent_t ev_k0[kNumBins] = {}, ev_k1[kNumBins] = {};
     DevStats *d_stats = nullptr;
     DevStats *h_stats = nullptr;  // pinned
-    DevBuf<int> row_flop, row_nnz, bins, cursor, crpt32_tmp;
+    DevBuf<int> row_flop, row_nnz, bins, cursor, spill, crpt32_tmp;
     DevBuf<long long> block_sums, crpt64;
     BinBounds bounds = default_bounds();
     int short_hash = kHashLin;
@@ -100,6 +100,7 @@ struct PlanCore {
     int *bins = nullptr;          // numeric bin lists (nullptr: identity over one bin)
     int *cursor = nullptr;        // four ints per A-nonzero index, long-row bin only
     int span = 0;                 // highest A-nonzero index + 1
+    int *spill = nullptr;         // bin-5 rows handed from the wave kernel to the block kernel
     int num_size[kNumBins] = {};
     int num_off[kNumBins] = {};
     int hash = kHashLin;  // hash family of the short-row bins
@@ -110,7 +111,7 @@ struct ops_plan {
     ops_workspace *ws = nullptr;
     PlanCore core;
     DevBuf<long long> crpt64;
-    DevBuf<int> bins, cursor;
+    DevBuf<int> bins, cursor, spill;
     int annz = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
@@ -303,7 +304,7 @@ void numeric_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, const Plan
     const bool ev = t && t->kernel_events;
     dispatch_bins(ws, s, pc.num_size, ev, [&](int b, hipStream_t ps) {
         launch_numeric_bin<V>(b, A, B, pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b], pc.crpt64, ccol,
-                              cval, pc.cursor, pc.span, pc.hash, ps);
+                              cval, pc.cursor, pc.span, pc.spill, pc.hash, ps);
     });
 }
 
@@ -409,6 +410,10 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
         pc.span = static_cast<int>(a_index_span(A, s));
         ws->cursor.ensure(4 * static_cast<size_t>(pc.span) + 4);
         pc.cursor = ws->cursor.p;
+    }
+    if (pc.num_size[kSpillBin] > 0) {
+        ws->spill.ensure(static_cast<size_t>(pc.num_size[kSpillBin]) + 1);
+        pc.spill = ws->spill.p;
     }
     if (t) t->allocate = now_s() - t0;
 
@@ -535,6 +540,7 @@ int ops_workspace_destroy(ops_workspace *ws)
     ws->row_nnz.release();
     ws->bins.release();
     ws->cursor.release();
+    ws->spill.release();
     ws->crpt32_tmp.release();
     ws->block_sums.release();
     ws->crpt64.release();
@@ -719,6 +725,10 @@ int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_p
             p->cursor.ensure(4 * static_cast<size_t>(p->core.span) + 4);
             p->core.cursor = p->cursor.p;
         }
+        if (p->core.num_size[kSpillBin] > 0) {
+            p->spill.ensure(static_cast<size_t>(p->core.num_size[kSpillBin]) + 1);
+            p->core.spill = p->spill.p;
+        }
         CK(hipStreamSynchronize(s));
         if (nnz_c) *nnz_c = p->core.nnz;
         return OPS_OK;
@@ -739,6 +749,7 @@ int ops_plan_destroy(ops_plan *plan)
     plan->crpt64.release();
     plan->bins.release();
     plan->cursor.release();
+    plan->spill.release();
     delete plan;
     return OPS_OK;
 }

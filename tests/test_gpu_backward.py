@@ -97,16 +97,17 @@ def test_autograd_function_and_finite_differences(gpu):
 
 def test_backward_long_rows_skipped_windows(gpu):
     """Rows of C above 4096 entries whose columns sit in two far-apart bands:
-    the dense-window walk visits only the windows that hold entries."""
+    the long-row walk visits only the windows that hold entries."""
     rng = np.random.default_rng(9)
-    N = 300000
+    N = 700000
     A = random_csr(16, N, np.r_[rng.integers(60, 90, 8), rng.integers(1, 3, 8)], rng)
     used = np.unique(A.col)
     lens = np.zeros(N, dtype=np.int64)
     lens[used] = rng.integers(60, 100, used.size)
     B = random_csr(N, 40000, lens, rng)
-    # columns [0, 20000) stay, [20000, 40000) move to [200000, 220000): monotone, so rows stay sorted
-    col = np.where(B.col < 20000, B.col, B.col + 180000).astype(np.int32)
+    # columns [0, 20000) stay, [20000, 40000) move to [600000, 620000): monotone, so rows stay sorted;
+    # the 2^18-column windows in between hold no entry of C and are skipped
+    col = np.where(B.col < 20000, B.col, B.col + 580000).astype(np.int32)
     B = G.HostCSR(N, N, B.rpt, col, B.val)
     check_grad(A, B, torch.float64)
     check_grad(A, B, torch.float32)
