@@ -163,6 +163,45 @@ template <typename Op>
 struct has_prep<Op, std::void_t<decltype(&Op::prep)>> : std::true_type {
 };
 
+// A consumer may also define
+//   void load4(int b_index, Item (&it)[4]) const;   // b_index % 4 == 0
+//   bool vec_ok() const;                            // B arrays 16-byte aligned
+// reading four consecutive B entries with 16-byte loads; the grouped walk
+// then takes long B rows four entries per lane per step.
+template <typename Op, typename = void>
+struct has_load4 : std::false_type {
+};
+template <typename Op>
+struct has_load4<Op, std::void_t<decltype(&Op::load4)>> : std::true_type {
+};
+
+__device__ __forceinline__ void load4_cols(const int *__restrict__ p, int (&c)[4])
+{
+    const int4 v = *reinterpret_cast<const int4 *>(p);
+    c[0] = v.x;
+    c[1] = v.y;
+    c[2] = v.z;
+    c[3] = v.w;
+}
+__device__ __forceinline__ void load4_vals(const float *__restrict__ p, float (&b)[4])
+{
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    b[0] = v.x;
+    b[1] = v.y;
+    b[2] = v.z;
+    b[3] = v.w;
+}
+__device__ __forceinline__ void load4_vals(const double *__restrict__ p, double (&b)[4])
+{
+    const double2 v0 = *reinterpret_cast<const double2 *>(p);
+    const double2 v1 = *reinterpret_cast<const double2 *>(p + 2);
+    b[0] = v0.x;
+    b[1] = v0.y;
+    b[2] = v1.x;
+    b[3] = v1.y;
+}
+__device__ __forceinline__ bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
 template <int NP, typename Op, typename V>
 __device__ __forceinline__ void apply_batch(Op &op, const typename Op::Item (&it)[NP], const V (&a)[NP],
                                             const int (&q)[NP], const bool (&ok)[NP])
@@ -214,14 +253,42 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
         const int ql = q < cnt ? lq : 0;
         V qa = V(0);
         if constexpr (WITH_A) qa = bperm(av, src);
+        const V a2[2] = {qa, qa};
+        const int q2[2] = {q, q};
+        if constexpr (has_load4<Op>::value && GS >= 16) {
+            if (op.vec_ok()) {
+                // Four consecutive entries per lane per step from the first
+                // 4-aligned entry; the unaligned head and the tail (at most
+                // three entries each) go to the part that takes step 0.
+                const int h_end = min(qb + ql, (qb + 3) & ~3);
+                const int nvec = (qb + ql - h_end) >> 2;
+                const int b_end = h_end + 4 * nvec;
+                if (!SLICE || part == 0) {
+                    typename Op::Item it[2];
+                    const bool ok[2] = {sub < h_end - qb, sub < qb + ql - b_end};
+                    if (ok[0]) it[0] = op.load(qb + sub);
+                    if (ok[1]) it[1] = op.load(b_end + sub);
+                    apply_batch<2>(op, it, a2, q2, ok);
+                }
+                const int v0 = SLICE ? sub + part * GS : sub;
+                const int vstep = SLICE ? nparts * GS : GS;
+                const V a4[4] = {qa, qa, qa, qa};
+                const int q4[4] = {q, q, q, q};
+                const bool ok[4] = {true, true, true, true};
+                for (int v = v0; v < nvec; v += vstep) {
+                    typename Op::Item it[4];
+                    op.load4(h_end + 4 * v, it);
+                    apply_batch<4>(op, it, a4, q4, ok);
+                }
+                continue;
+            }
+        }
         for (int t = t0; t < ql; t += tstep) {
             typename Op::Item it[2];
             const bool ok[2] = {true, t + GS < ql};
             it[0] = op.load(qb + t);
             if (ok[1]) it[1] = op.load(qb + t + GS);
-            const V aa[2] = {qa, qa};
-            const int qq[2] = {q, q};
-            apply_batch<2>(op, it, aa, qq, ok);
+            apply_batch<2>(op, it, a2, q2, ok);
         }
     }
 }
@@ -459,6 +526,210 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
         }
         if constexpr (has_chunk_end<Op>::value) op.chunk_end(e, e < a_end, shared);
     }
+}
+
+// OR of `v` over the 64 lanes of the wave, returned to every lane.  DPP
+// row shifts reduce each row of 16 lanes into its lane 15, the row
+// broadcasts carry rows 0-2 into lane 63, and lane 63 is read back as a
+// scalar: no LDS instruction, so nothing waits behind outstanding LDS work.
+__device__ __forceinline__ unsigned wave_or(unsigned v)
+{
+    int x = static_cast<int>(v);
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); // row_bcast:15
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); // row_bcast:31
+    return static_cast<unsigned>(__builtin_amdgcn_readlane(x, kWave - 1));
+}
+
+// Block-wide stage of one batch of up to BS A-nonzeros for the block flat
+// walk: the A-nonzeros with a non-empty B range, in order, with the
+// exclusive scan of their range lengths (excl[n] = total products) and per
+// entry the B index offset (B index = product position + off), the
+// A-nonzero index and its value.
+template <int BS, typename V>
+struct BlockStage {
+    static constexpr int NW = BS / kWave;
+    struct alignas(16) Ent {
+        int off;
+        int e;
+        V av;
+    };
+    int excl[BS + 1];
+    Ent ent[BS];
+    int wsum[2][NW];
+};
+
+// Stages A-nonzeros [c, min(c + BS, a_end)) (one per thread) into `st`:
+// each thread computes its own A-nonzero's range, so a range is computed
+// once per batch whatever the number of waves.  Returns the number of
+// staged (non-empty) ranges; `total` gets the batch's product count.
+// Every thread of the block calls it; it ends with a barrier.  The
+// caller's previous walk of `st` must be complete (a barrier) before.
+template <int BS, typename V, typename Range>
+__device__ __forceinline__ int block_stage(int c, int a_end, const int *__restrict__ acol, const V *__restrict__ aval,
+                                           const Range &range, BlockStage<BS, V> &st, int &total)
+{
+    constexpr int NW = BS / kWave;
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid / kWave;
+    const int e = c + tid;
+    int bs = 0, len = 0;
+    V av = V(0);
+    if (e < a_end) {
+        range(e, acol[e], bs, len);
+        av = aval[e];
+    }
+    const int f = len > 0 ? 1 : 0;
+    const int il = unit_incl_scan<kWave>(len);
+    const int iff = unit_incl_scan<kWave>(f);
+    if (lane == kWave - 1) {
+        st.wsum[0][wave] = il;
+        st.wsum[1][wave] = iff;
+    }
+    __syncthreads();
+    int pl = 0, pf = 0, tl = 0, tf = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const int a = st.wsum[0][w], b = st.wsum[1][w];
+        if (w < wave) {
+            pl += a;
+            pf += b;
+        }
+        tl += a;
+        tf += b;
+    }
+    if (f) {
+        const int ex = pl + il - len;
+        const int ix = pf + iff - f;
+        st.excl[ix] = ex;
+        st.ent[ix] = typename BlockStage<BS, V>::Ent{bs - ex, e, av};
+    }
+    if (tid == 0) st.excl[tf] = tl;
+    __syncthreads();
+    total = tl;
+    return tf;
+}
+
+// Walks every product of a staged batch with all waves of the block: wave
+// w takes one contiguous slice of about total/NW products, and its lanes
+// take 64 consecutive products per step (coalesced B reads whatever the
+// range lengths).  The owner of each product comes from the step's range
+// starts: lane l reads the start of range qa+1+l (qa owns the step's first
+// product), the starts inside the step form a 64-bit mask (wave_or), and a
+// lane's owner is qa plus the starts at or before it.  This costs the same
+// few instructions per 64 products for any mix of long and short ranges and
+// needs no per-product search.  NP steps are gathered at once and their B
+// loads stay in flight while the previous steps' updates run.  Ends with a
+// barrier (the stage may then be overwritten).
+template <int BS, bool WITH_A, typename V, typename Op>
+__device__ __forceinline__ void block_walk(const BlockStage<BS, V> &st, int n, int total, Op &op)
+{
+    constexpr int NW = BS / kWave;
+    constexpr int NP = 4;
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    const int seg = (((total + NW - 1) / NW) + kWave - 1) & ~(kWave - 1);
+    const int s_beg = wave * seg;
+    const int s_end = min(total, s_beg + seg);
+    if (s_beg < s_end) {
+        // Owner of s_beg: largest q < n with excl[q] <= s_beg, by a
+        // 64-ary search in two rounds (n <= 64 * 64).
+        static_assert(BS <= kWave * kWave, "two-round owner search");
+        const int g1 = (n + kWave - 1) / kWave;
+        int q = lane * g1;
+        int cnt = __popcll(__ballot(q < n && st.excl[q] <= s_beg));
+        const int lo = (cnt - 1) * g1;
+        q = lo + lane;
+        cnt = __popcll(__ballot(lane < g1 && q < n && st.excl[q] <= s_beg));
+        int qa = lo + cnt - 1;
+        const unsigned long long below = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
+        auto gather = [&](int b0, int (&bi)[NP], V (&aa)[NP], int (&qq)[NP], bool (&ok)[NP]) {
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                const int sb = b0 + s * kWave;
+                ok[s] = false;
+                if (sb < s_end) {
+                    const int qc = qa + 1 + lane;
+                    const int pos = qc <= n ? st.excl[qc] - sb : kWave + 1;
+                    const unsigned lo32 = pos > 0 && pos < 32 ? 1u << pos : 0u;
+                    const unsigned hi32 = pos >= 32 && pos < kWave ? 1u << (pos - 32) : 0u;
+                    const unsigned long long m = (static_cast<unsigned long long>(wave_or(hi32)) << 32) | wave_or(lo32);
+                    const int qo = qa + __popcll(m & below);
+                    qa += __popcll(m) + (__ballot(pos == kWave) != 0ull ? 1 : 0);
+                    const int p = sb + lane;
+                    ok[s] = p < s_end;
+                    if (ok[s]) {
+                        const typename BlockStage<BS, V>::Ent en = st.ent[qo];
+                        bi[s] = p + en.off;
+                        aa[s] = en.av;
+                        qq[s] = en.e;
+                    }
+                }
+            }
+        };
+        const int stride = NP * kWave;
+        int base = s_beg;
+        int bb[NP], qq[NP];
+        V aa[NP];
+        bool ok[NP];
+        typename Op::Item it[NP];
+        gather(base, bb, aa, qq, ok);
+#pragma unroll
+        for (int s = 0; s < NP; s++)
+            if (ok[s]) it[s] = op.load(bb[s]);
+        while (true) {
+            const int nb = base + stride;
+            const bool more = nb < s_end;
+            int bb2[NP], qq2[NP];
+            V aa2[NP];
+            bool ok2[NP];
+            typename Op::Item it2[NP];
+            if (more) {
+                gather(nb, bb2, aa2, qq2, ok2);
+#pragma unroll
+                for (int s = 0; s < NP; s++)
+                    if (ok2[s]) it2[s] = op.load(bb2[s]);
+            }
+            if constexpr (!WITH_A) {
+#pragma unroll
+                for (int s = 0; s < NP; s++) aa[s] = V(0);
+            }
+            apply_batch<NP>(op, it, aa, qq, ok);
+            if (!more) break;
+            base = nb;
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                it[s] = it2[s];
+                aa[s] = aa2[s];
+                qq[s] = qq2[s];
+                ok[s] = ok2[s];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Calls the consumer for every product of A row segment [a_beg, a_end) with
+// the whole block, in batches of BS A-nonzeros (block_stage + block_walk).
+// The consumer's q argument is the A-nonzero index.  Returns true when the
+// row fit in one batch: `st` then still holds it for another walk over the
+// same ranges (block_walk with the returned n and total).
+template <int BS, bool WITH_A, typename V, typename Range, typename Op>
+__device__ __forceinline__ bool block_for_each_product(int a_beg, int a_end, const int *__restrict__ acol,
+                                                       const V *__restrict__ aval, const Range &range, Op &op,
+                                                       BlockStage<BS, V> &st, int &n, int &total)
+{
+    n = 0;
+    total = 0;
+    for (int c = a_beg; c < a_end; c += BS) {
+        n = block_stage<BS, V>(c, a_end, acol, aval, range, st, total);
+        if (total > 0) block_walk<BS, WITH_A, V>(st, n, total, op);
+    }
+    return a_end - a_beg <= BS;
 }
 
 // Cursor-carrying B range for window [w0, wend): lo is the first entry of

@@ -499,11 +499,10 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     __shared__ int s_scan[kLongBS / kWave + 1];
     __shared__ int s_min[kLongBS / kWave];
     __shared__ int s_cut;
-    __shared__ FlatStage<kWave, V> s_stage[kLongBS / kWave];
+    __shared__ BlockStage<kLongBS, V> s_bst;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int wb0 = tid * WPT;
-    FlatStage<kWave, V> *const stage = s_stage + wave;
     int *const cs[2] = {cursor, cursor + span};
     int *const cz[2] = {cursor + 2 * static_cast<long long>(span), cursor + 3 * static_cast<long long>(span)};
 
@@ -535,10 +534,12 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             const bool last = wend >= ncols;
             // 1. mark
             int nxt = INT_MAX;
+            int sn = 0, stot = 0;
+            bool one = false;
             {
                 const CursorRange range{brpt, bcol, cs[p], cs[p ^ 1], wend, first, last, &nxt};
                 RankMark<V> op{bcol, br, w0};
-                for_each_product<kWave, false, V>(a_beg, a_end, wave, NU, acol, aval, range, op, stage);
+                one = block_for_each_product<kLongBS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn, stot);
             }
             nxt = unit_min<kWave>(nxt);
             if (lane_id() == 0) s_min[wave] = nxt;
@@ -584,14 +585,21 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     // At most two sub-windows: walk the window's products over
                     // the ranges step 1 recorded and skip the other sub-window's
                     // ranks, instead of locating a cut in every B row.
-                    const StoredRange range{brpt, cs[p], cs[p ^ 1], first};
-                    for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op, stage);
+                    // A row that fit in one batch is still staged from step 1.
+                    if (one) {
+                        if (stot > 0) block_walk<kLongBS, true, V>(s_bst, sn, stot, op);
+                    } else {
+                        const StoredRange range{brpt, cs[p], cs[p ^ 1], first};
+                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                                                                 stot);
+                    }
                 } else {
                     int unused = INT_MAX;
                     const int *cin = sub == 0 ? cs[p] : cz[(sub - 1) & 1];
                     const CursorRange range{brpt,       bcol, cin, cz[sub & 1], cb, first && sub == 0,
                                             last && cb == wend, &unused};
-                    for_each_product<kWave, true, V>(a_beg, a_end, wave, NU, acol, aval, range, op, stage);
+                    int n2, t2;
+                    block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                 }
                 __syncthreads();
                 // 4. write: the values are already in rank order, so they
