@@ -732,6 +732,28 @@ __device__ __forceinline__ bool block_for_each_product(int a_beg, int a_end, con
     return a_end - a_beg <= BS;
 }
 
+// First index in [lo, t) of the sorted B row segment whose column is >=
+// bound (t if none): galloping from lo, then a binary search, so the cost
+// grows with the distance from lo rather than with the row length.
+__device__ __forceinline__ int gallop_lower(const int *__restrict__ bcol, int lo, int t, int bound)
+{
+    if (lo >= t || bcol[lo] >= bound) return lo;
+    int prev = lo, step = 1;
+    while (prev + step < t && bcol[prev + step] < bound) {
+        prev += step;
+        step <<= 1;
+    }
+    int l = prev + 1, h = min(prev + step, t);
+    while (l < h) {
+        const int m = (l + h) >> 1;
+        if (bcol[m] < bound)
+            l = m + 1;
+        else
+            h = m;
+    }
+    return l;
+}
+
 // Cursor-carrying B range for window [w0, wend): lo is the first entry of
 // the B row not consumed by earlier windows; hi the first entry >= wend,
 // found by galloping from lo.  Tracks the smallest column beyond the window
@@ -749,29 +771,59 @@ struct CursorRange {
     {
         const int t = brpt[k + 1];
         const int lo = first ? brpt[k] : cur_in[e];
-        int hi;
-        if (last) {
-            hi = t;
-        } else if (lo >= t || bcol[lo] >= wend) {
-            hi = lo;
-        } else {
-            int prev = lo, step = 1;
-            while (prev + step < t && bcol[prev + step] < wend) {
-                prev += step;
-                step <<= 1;
-            }
-            int l = prev + 1, h = min(prev + step, t);
-            while (l < h) {
-                const int m = (l + h) >> 1;
-                if (bcol[m] < wend)
-                    l = m + 1;
-                else
-                    h = m;
-            }
-            hi = l;
-        }
+        const int hi = last ? t : gallop_lower(bcol, lo, t, wend);
         cur_out[e] = hi;
         if (hi < t) *next_col = min(*next_col, bcol[hi]);
+        bs = lo;
+        len = hi - lo;
+    }
+};
+
+// B range of an A-nonzero inside column window wi, from the window-cut
+// table (entries [cut[k][wi], cut[k][wi + 1]) of B row k; cut[k][0] is the
+// row start and cut[k][nwin] the row end): two independent loads instead of
+// a search.  Tracks the smallest column beyond the window when next_col is
+// given.
+struct WinCutRange {
+    const int *__restrict__ wcut;
+    const int *__restrict__ bcol;
+    int stride;  // windows + 1
+    int wi;
+    int *next_col;
+    __device__ __forceinline__ void operator()(int /*e*/, int k, int &bs, int &len) const
+    {
+        const int *c = wcut + static_cast<long long>(k) * stride;
+        const int lo = c[wi], hi = c[wi + 1];
+        if (next_col) {
+            const int t = c[stride - 1];
+            if (hi < t) *next_col = min(*next_col, bcol[hi]);
+        }
+        bs = lo;
+        len = hi - lo;
+    }
+};
+
+// Sub-window [.., cb) of column window wi: the range starts at the window's
+// cut (first sub-window) or where the previous sub-window stopped, and ends
+// at the first column >= cb, found by galloping no further than the
+// window's end cut.
+struct CutCursorRange {
+    const int *__restrict__ wcut;
+    const int *__restrict__ bcol;
+    int stride;
+    int wi;
+    const int *__restrict__ cur_in;
+    int *__restrict__ cur_out;
+    int cb;
+    bool first;
+    bool to_end;  // cb is the window end
+    __device__ __forceinline__ void operator()(int e, int k, int &bs, int &len) const
+    {
+        const int *c = wcut + static_cast<long long>(k) * stride;
+        const int t = c[wi + 1];
+        const int lo = first ? c[wi] : cur_in[e];
+        const int hi = to_end ? t : gallop_lower(bcol, lo, t, cb);
+        cur_out[e] = hi;
         bs = lo;
         len = hi - lo;
     }

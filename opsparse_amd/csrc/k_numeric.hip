@@ -460,7 +460,7 @@ constexpr int kLongBS = 1024;
 // wave has already advanced.
 template <typename V>
 struct RankWin {
-    static constexpr int WB = 1 << 18;                          // columns per bitmap window
+    static constexpr int WB = kRankWindow;                      // columns per bitmap window
     static constexpr int WORDS = WB / 32;                       // bitmap words (32 KB)
     static constexpr int VCAP = sizeof(V) == 8 ? 8192 : 16384;  // values per accumulate pass (64 KB)
     static constexpr size_t lds = sizeof(V) * VCAP + sizeof(uint2) * WORDS;
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                                                       const int *__restrict__ bcol, const V *__restrict__ bval,
                                                       const int *__restrict__ rows, int nrows, int ncols,
                                                       const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                      V *__restrict__ cval, int *__restrict__ cursor, int span)
+                                                      V *__restrict__ cval, LongScratch ls)
 {
     using R = RankWin<V>;
     constexpr int NU = kLongBS / kWave;
@@ -503,8 +503,11 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int wb0 = tid * WPT;
+    int *const cursor = ls.cursor;
+    const long long span = ls.span;
     int *const cs[2] = {cursor, cursor + span};
-    int *const cz[2] = {cursor + 2 * static_cast<long long>(span), cursor + 3 * static_cast<long long>(span)};
+    int *const cz[2] = {cursor + 2 * span, cursor + 3 * span};
+    const int cstride = ls.nwin + 1;
 
     for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
     __syncthreads();
@@ -536,10 +539,18 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             int nxt = INT_MAX;
             int sn = 0, stot = 0;
             bool one = false;
+            const int wi = w0 / R::WB;
             {
-                const CursorRange range{brpt, bcol, cs[p], cs[p ^ 1], wend, first, last, &nxt};
                 RankMark<V> op{bcol, br, w0};
-                one = block_for_each_product<kLongBS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn, stot);
+                if (ls.wcut) {
+                    const WinCutRange range{ls.wcut, bcol, cstride, wi, &nxt};
+                    one = block_for_each_product<kLongBS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                                                                    stot);
+                } else {
+                    const CursorRange range{brpt, bcol, cs[p], cs[p ^ 1], wend, first, last, &nxt};
+                    one = block_for_each_product<kLongBS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                                                                    stot);
+                }
             }
             nxt = unit_min<kWave>(nxt);
             if (lane_id() == 0) s_min[wave] = nxt;
@@ -588,18 +599,28 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     // A row that fit in one batch is still staged from step 1.
                     if (one) {
                         if (stot > 0) block_walk<kLongBS, true, V>(s_bst, sn, stot, op);
+                    } else if (ls.wcut) {
+                        const WinCutRange range{ls.wcut, bcol, cstride, wi, nullptr};
+                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                                                                 stot);
                     } else {
                         const StoredRange range{brpt, cs[p], cs[p ^ 1], first};
                         block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     }
                 } else {
-                    int unused = INT_MAX;
-                    const int *cin = sub == 0 ? cs[p] : cz[(sub - 1) & 1];
-                    const CursorRange range{brpt,       bcol, cin, cz[sub & 1], cb, first && sub == 0,
-                                            last && cb == wend, &unused};
                     int n2, t2;
-                    block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
+                    if (ls.wcut) {
+                        const CutCursorRange range{ls.wcut,    bcol, cstride,    wi, cz[(sub - 1) & 1], cz[sub & 1],
+                                                   cb,         sub == 0, cb == wend};
+                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
+                    } else {
+                        int unused = INT_MAX;
+                        const int *cin = sub == 0 ? cs[p] : cz[(sub - 1) & 1];
+                        const CursorRange range{brpt,       bcol, cin, cz[sub & 1], cb, first && sub == 0,
+                                                last && cb == wend, &unused};
+                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
+                    }
                 }
                 __syncthreads();
                 // 4. write: the values are already in rank order, so they
@@ -697,7 +718,7 @@ void numeric_init_attributes()
 
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, int *cursor, int span, int *spill, int hash, hipStream_t s)
+                        int *ccol, V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
@@ -723,14 +744,55 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows
         default:
             k_num_rank<V><<<nrows, kLongBS, RankWin<V>::lds, s>>>(
                 A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
-                B.N, crpt, ccol, cval, cursor, span);
+                B.N, crpt, ccol, cval, ls);
             break;
     }
 }
 
 template void launch_numeric_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
-                                         double *, int *, int, int *, int, hipStream_t);
+                                         double *, const LongScratch &, int *, int, hipStream_t);
 template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
-                                        float *, int *, int, int *, int, hipStream_t);
+                                        float *, const LongScratch &, int *, int, hipStream_t);
+
+namespace {
+
+__global__ void k_window_cuts(const int *__restrict__ brpt, const int *__restrict__ bcol, int nb, int nwin,
+                              int *__restrict__ wcut)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int stride = nwin + 1;
+    if (i >= static_cast<long long>(nb) * stride) return;
+    const int k = static_cast<int>(i / stride);
+    const int w = static_cast<int>(i % stride);
+    const int lo = brpt[k], hi = brpt[k + 1];
+    int v;
+    if (w == 0) {
+        v = lo;
+    } else if (w == nwin) {
+        v = hi;
+    } else {
+        const int bound = w * kRankWindow;
+        int l = lo, h = hi;
+        while (l < h) {
+            const int m = (l + h) >> 1;
+            if (bcol[m] < bound)
+                l = m + 1;
+            else
+                h = m;
+        }
+        v = l;
+    }
+    wcut[i] = v;
+}
+
+}  // namespace
+
+void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s)
+{
+    const long long n = static_cast<long long>(B.M) * (nwin + 1);
+    if (n <= 0) return;
+    constexpr int BS = 256;
+    k_window_cuts<<<static_cast<unsigned>((n + BS - 1) / BS), BS, 0, s>>>(B.rpt, B.col, B.M, nwin, wcut);
+}
 
 }  // namespace opsamd

@@ -109,10 +109,29 @@ size_t symbolic_long_lds();
 // Numeric bin run one row per wave first (rows too wide for its bitmap are
 // passed on to the block kernel through `spill`).
 constexpr int kSpillBin = 5;
+
+// Column window of the long-row bitmap kernels (numeric and backward).
+constexpr int kRankWindow = 1 << 18;
+// Largest window-cut table built (bytes); beyond it the long-row kernels
+// locate window boundaries by galloping in each B row instead.
+constexpr long long kMaxCutBytes = 1ll << 30;
+
+// Scratch of the long-row bin.
+struct LongScratch {
+    int *cursor = nullptr;      // four ints per A-nonzero index
+    int span = 0;               // highest A-nonzero index + 1
+    const int *wcut = nullptr;  // window cuts of every B row, or nullptr
+    int nwin = 0;               // column windows of kRankWindow (wcut has nwin + 1 per B row)
+};
+
+// wcut[k * (nwin + 1) + w] = first entry of B row k with column >= w *
+// kRankWindow (w = 0: the row start, w = nwin: the row end).
+void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s);
+
 // spill: scratch of (rows of bin kSpillBin + 1) ints, or nullptr (block kernel only).
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, int *cursor, int span, int *spill, int hash, hipStream_t s);
+                        int *ccol, V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s);
 // One-time kernel attribute setup (dynamic LDS limits).
 void numeric_init_attributes();
 void symbolic_init_attributes();

@@ -12,10 +12,12 @@
 //  * C's row offsets are computed in 64 bits, so products above 2^31 nonzeros
 //    are supported by the 64-bit entry points.
 #include <hip/hip_runtime.h>
+
 #include <time.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -85,7 +87,7 @@ struct ops_workspace {
     hipEvent_t ev_k0[kNumBins] = {}, ev_k1[kNumBins] = {};
     DevStats *d_stats = nullptr;
     DevStats *h_stats = nullptr;  // pinned
-    DevBuf<int> row_flop, row_nnz, bins, cursor, spill, crpt32_tmp;
+    DevBuf<int> row_flop, row_nnz, bins, cursor, spill, wcut, crpt32_tmp;
     DevBuf<long long> block_sums, crpt64;
     BinBounds bounds = default_bounds();
     int short_hash = kHashLin;
@@ -98,8 +100,7 @@ struct PlanCore {
     long long nnz = 0;
     long long *crpt64 = nullptr;  // M+1, device
     int *bins = nullptr;          // numeric bin lists (nullptr: identity over one bin)
-    int *cursor = nullptr;        // four ints per A-nonzero index, long-row bin only
-    int span = 0;                 // highest A-nonzero index + 1
+    LongScratch ls;               // long-row bin only
     int *spill = nullptr;         // bin-5 rows handed from the wave kernel to the block kernel
     int num_size[kNumBins] = {};
     int num_off[kNumBins] = {};
@@ -111,7 +112,7 @@ struct ops_plan {
     ops_workspace *ws = nullptr;
     PlanCore core;
     DevBuf<long long> crpt64;
-    DevBuf<int> bins, cursor, spill;
+    DevBuf<int> bins, cursor, spill, wcut;
     int annz = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
@@ -304,7 +305,7 @@ void numeric_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, const Plan
     const bool ev = t && t->kernel_events;
     dispatch_bins(ws, s, pc.num_size, ev, [&](int b, hipStream_t ps) {
         launch_numeric_bin<V>(b, A, B, pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b], pc.crpt64, ccol,
-                              cval, pc.cursor, pc.span, pc.spill, pc.hash, ps);
+                              cval, pc.ls, pc.spill, pc.hash, ps);
     });
 }
 
@@ -338,6 +339,36 @@ long long a_index_span(const ops_csr *A, hipStream_t s)
     CK(hipMemcpyAsync(&last, A->rpt + A->M, sizeof(int), hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
     return last;
+}
+
+// Scratch of the numeric phase: the spill list of bin kSpillBin and, for the
+// long-row bin, the per-A-nonzero cursors and the window-cut table of B
+// (built here on stream s when B has more than one column window and the
+// table stays within kMaxCutBytes; it depends on B's pattern only).
+void numeric_scratch(PlanCore &pc, const ops_csr *A, const ops_csr *B, DevBuf<int> &cursor, DevBuf<int> &spill,
+                     DevBuf<int> &wcut, hipStream_t s)
+{
+    if (pc.num_size[kNumBins - 1] > 0) {
+        pc.ls.span = static_cast<int>(a_index_span(A, s));
+        cursor.ensure(4 * static_cast<size_t>(pc.ls.span) + 4);
+        pc.ls.cursor = cursor.p;
+        const int nwin = static_cast<int>((static_cast<long long>(B->N) + kRankWindow - 1) / kRankWindow);
+        const long long bytes = static_cast<long long>(sizeof(int)) * B->M * (nwin + 1);
+        // OPSAMD_CUT_TABLE=0 turns the table off (galloping path), for tests.
+        const char *env = std::getenv("OPSAMD_CUT_TABLE");
+        const bool off = env && env[0] == '0';
+        if (nwin > 1 && bytes <= kMaxCutBytes && !off) {
+            wcut.ensure(static_cast<size_t>(B->M) * (nwin + 1));
+            launch_window_cuts(as_in(B), nwin, wcut.p, s);
+            CK(hipGetLastError());
+            pc.ls.wcut = wcut.p;
+            pc.ls.nwin = nwin;
+        }
+    }
+    if (pc.num_size[kSpillBin] > 0) {
+        spill.ensure(static_cast<size_t>(pc.num_size[kSpillBin]) + 1);
+        pc.spill = spill.p;
+    }
 }
 
 template <typename V>
@@ -406,15 +437,7 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
         C32->col = ccol;
         C32->val = cval;
     }
-    if (pc.num_size[kNumBins - 1] > 0) {
-        pc.span = static_cast<int>(a_index_span(A, s));
-        ws->cursor.ensure(4 * static_cast<size_t>(pc.span) + 4);
-        pc.cursor = ws->cursor.p;
-    }
-    if (pc.num_size[kSpillBin] > 0) {
-        ws->spill.ensure(static_cast<size_t>(pc.num_size[kSpillBin]) + 1);
-        pc.spill = ws->spill.p;
-    }
+    numeric_scratch(pc, A, B, ws->cursor, ws->spill, ws->wcut, s);
     if (t) t->allocate = now_s() - t0;
 
     t0 = now_s();
@@ -541,6 +564,7 @@ int ops_workspace_destroy(ops_workspace *ws)
     ws->bins.release();
     ws->cursor.release();
     ws->spill.release();
+    ws->wcut.release();
     ws->crpt32_tmp.release();
     ws->block_sums.release();
     ws->crpt64.release();
@@ -720,15 +744,7 @@ int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_p
         p->crpt64.ensure(static_cast<size_t>(M) + 1);
         p->bins.ensure(static_cast<size_t>(M) + 1);
         symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr);
-        if (p->core.num_size[kNumBins - 1] > 0) {
-            p->core.span = static_cast<int>(a_index_span(A, s));
-            p->cursor.ensure(4 * static_cast<size_t>(p->core.span) + 4);
-            p->core.cursor = p->cursor.p;
-        }
-        if (p->core.num_size[kSpillBin] > 0) {
-            p->spill.ensure(static_cast<size_t>(p->core.num_size[kSpillBin]) + 1);
-            p->core.spill = p->spill.p;
-        }
+        numeric_scratch(p->core, A, B, p->cursor, p->spill, p->wcut, s);
         CK(hipStreamSynchronize(s));
         if (nnz_c) *nnz_c = p->core.nnz;
         return OPS_OK;
@@ -750,6 +766,7 @@ int ops_plan_destroy(ops_plan *plan)
     plan->bins.release();
     plan->cursor.release();
     plan->spill.release();
+    plan->wcut.release();
     delete plan;
     return OPS_OK;
 }

@@ -193,6 +193,32 @@ def test_long_rows_many_rank_subwindows(gpu):
     run(A, B, wide=True)
 
 
+def test_long_rows_multi_window_cut_table_and_gallop(gpu, monkeypatch):
+    """Long rows over 6 column windows of 2^18 with windows 1, 3 and 4
+    empty: the mark pass skips them, window 2 needs several value
+    sub-windows.  Run with the window-cut table of B (default) and with the
+    table turned off (window boundaries found by galloping in each B row)."""
+    rng = np.random.default_rng(14)
+    W = 1 << 18
+    N = 5 * W + 7
+    K = 1200
+    # columns drawn over 3 windows' worth, then placed in windows 0, 2 and 5
+    lens = np.r_[rng.integers(250, 330, K - 8), rng.integers(1, 4, 8)]
+    B3 = random_csr(K, 3 * W, lens, rng)
+    col = B3.col.astype(np.int64)
+    wmap = np.array([0, 2, 5], dtype=np.int64)
+    col = wmap[col // W] * W + col % W
+    col = np.minimum(col, N - 1)
+    B = G.HostCSR(K, N, B3.rpt.copy(), col.astype(np.int32), B3.val.copy())
+    A = random_csr(5, K, np.r_[rng.integers(300, 420, 3), 1, 0], rng)
+    for mode in ("1", "0"):
+        monkeypatch.setenv("OPSAMD_CUT_TABLE", mode)
+        log(f"multi-window cut table={mode}")
+        run(A, B)
+        run(A, B, np.float32)
+        run(A, B, wide=True)
+
+
 def test_block_bins_rank_and_hash_paths(gpu):
     """Rows of 256..4096 entries: a narrow column span takes the bitmap-rank
     path of the block bins, a span over 2^21 columns the hash + sort path."""
