@@ -562,7 +562,8 @@ struct BlockStage {
     int wsum[2][NW];
 };
 
-// Stages A-nonzeros [c, min(c + BS, a_end)) (one per thread) into `st`:
+// Stages A-nonzeros [c, min(c + BS, a_end)) (one per thread) into `st`
+// (aval may be nullptr: the staged values are then 0):
 // each thread computes its own A-nonzero's range, so a range is computed
 // once per batch whatever the number of waves.  Returns the number of
 // staged (non-empty) ranges; `total` gets the batch's product count.
@@ -581,7 +582,7 @@ __device__ __forceinline__ int block_stage(int c, int a_end, const int *__restri
     V av = V(0);
     if (e < a_end) {
         range(e, acol[e], bs, len);
-        av = aval[e];
+        if (aval) av = aval[e];
     }
     const int f = len > 0 ? 1 : 0;
     const int il = unit_incl_scan<kWave>(len);

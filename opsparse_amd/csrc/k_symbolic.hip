@@ -148,9 +148,8 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
 {
     extern __shared__ unsigned bm[];
     __shared__ int s_red[kLongBS / kWave];
+    __shared__ BlockStage<kLongBS, double> s_bst;
     const int tid = threadIdx.x;
-    const int wave = tid / kWave;
-    constexpr int NW = kLongBS / kWave;
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
         const int r = rows ? rows[ridx] : ridx;
         const int a_beg = arpt[r], a_end = arpt[r + 1];
@@ -162,10 +161,10 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
             __syncthreads();
             const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols};
             BitmapInsert op{bcol, bm, w0};
-            for_each_product<kWave, false, double, true>(a_beg, a_end, wave, NW, acol, static_cast<const double *>(nullptr),
-                                                   range, op);
+            int sn, stot;
+            block_for_each_product<kLongBS, false, double>(a_beg, a_end, acol, static_cast<const double *>(nullptr),
+                                                           range, op, s_bst, sn, stot);
             cnt += op.cnt;
-            __syncthreads();
         }
         const int tot = block_sum<kLongBS>(cnt, s_red);
         if (tid == 0) row_nnz[r] = tot;
