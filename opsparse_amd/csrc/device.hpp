@@ -544,6 +544,21 @@ __device__ __forceinline__ unsigned wave_or(unsigned v)
     return static_cast<unsigned>(__builtin_amdgcn_readlane(x, kWave - 1));
 }
 
+// Inclusive prefix sum over the 64 lanes of the wave: Hillis-Steele steps
+// by DPP row shifts inside each row of 16 lanes, then the row broadcasts
+// add row 0's total to row 1 and row 2's to row 3, and rows 0-1's total to
+// rows 2-3.  No LDS instruction.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
 // Block-wide stage of one batch of up to BS A-nonzeros for the block flat
 // walk: the A-nonzeros with a non-empty B range, in order, with the
 // exclusive scan of their range lengths (excl[n] = total products) and per
@@ -585,8 +600,8 @@ __device__ __forceinline__ int block_stage(int c, int a_end, const int *__restri
         if (aval) av = aval[e];
     }
     const int f = len > 0 ? 1 : 0;
-    const int il = unit_incl_scan<kWave>(len);
-    const int iff = unit_incl_scan<kWave>(f);
+    const int il = wave_incl_scan(len);
+    const int iff = wave_incl_scan(f);
     if (lane == kWave - 1) {
         st.wsum[0][wave] = il;
         st.wsum[1][wave] = iff;

@@ -14,7 +14,7 @@
 // are compacted and bitonic-sorted in registers across their unit
 // (cross-lane ds_bpermute) and rows up to 4096 are bitonic-sorted in LDS.
 // Rows above 4096 entries use no hash and no sort: a bitmap of the row's
-// columns per 2^18-column window gives every column its rank (its position
+// columns per 2^17-column window gives every column its rank (its position
 // in the sorted output), values accumulate at that rank in LDS, and the
 // window is written out in order (k_num_rank).
 #include <algorithm>
@@ -432,37 +432,42 @@ __global__ void k_zero_count(int *p)
 constexpr int kLongBS = 1024;
 
 // Bin 7: rows above 4096 entries, bitmap-rank windows.  The columns are
-// split into windows of WB = 2^18; per window:
+// split into windows of WB = 2^17; per window:
 //  1. mark: every product of the window sets its column's bit in an LDS
 //     bitmap (read first, atomic OR only for a clear bit);
-//  2. rank: a block scan of the per-word popcounts gives each word the
-//     number of the row's columns before it, so the position of column c in
-//     the window's sorted output is rk[word] + popcount(lower bits);
+//  2. rank: a scan of the per-word popcounts (DPP inside each wave, wave
+//     totals through LDS) gives each word the number of the row's columns
+//     before it, so the position of column c in the window's sorted output
+//     is rk[word] + popcount(lower bits);
 //  3. accumulate: the products are walked again and a*b is added to a
 //     compact LDS value array at that position.  When the window holds more
 //     than VCAP columns, this pass runs over column sub-windows of at most
 //     VCAP columns (cut at the column of rank r0 + VCAP).  With one or two
-//     sub-windows each walk reuses the B ranges step 1 found and skips the
-//     ranks of the other sub-window; with more, each sub-window locates its
-//     own B ranges with cursors, so no product is visited more than twice;
+//     sub-windows each walk reuses the B ranges step 1 found (still staged
+//     in LDS when the A row fits one batch) and skips the ranks of the other
+//     sub-window; with more, each sub-window locates its own B ranges with
+//     cursors, so no product is visited more than twice;
 //  4. write: values (already in rank order) and columns (placed by rank in
 //     LDS) are copied out contiguously.
 // No sort is needed and the value array only holds columns that exist, so a
-// row of 30K entries over 1M columns takes 4 bitmap windows instead of 64
+// row of 30K entries over 1M columns takes 8 bitmap windows instead of 64
 // dense ones.  Windows with no product are skipped (the next window starts
-// at the smallest column beyond the current one).
+// at the smallest column beyond the current one).  Walks are block-wide
+// (block_for_each_product): each batch of A-nonzeros has its B ranges
+// computed once, one per thread.
 //
-// cursor[] holds four arrays of span ints (span = highest A-nonzero index +
-// 1): two hold the window-start positions of the A-nonzeros' B rows and
-// alternate between windows, two alternate between the sub-windows of
-// step 3.  A walk never writes the array it reads, so waves that share an A
-// chunk (and compute the same positions) cannot see a position another
-// wave has already advanced.
+// A window's B ranges come from the window-cut table of B when one is given
+// (ls.wcut: the first entry of every B row at every window boundary);
+// otherwise from cursor[], four arrays of span ints (span = highest
+// A-nonzero index + 1): two hold the window-start positions of the
+// A-nonzeros' B rows and alternate between windows (the end found by
+// galloping), two alternate between the sub-windows of step 3 (with or
+// without the table).
 template <typename V>
 struct RankWin {
     static constexpr int WB = kRankWindow;                      // columns per bitmap window
     static constexpr int WORDS = WB / 32;                       // bitmap words (32 KB)
-    static constexpr int VCAP = sizeof(V) == 8 ? 8192 : 16384;  // values per accumulate pass (64 KB)
+    static constexpr int VCAP = sizeof(V) == 8 ? 12288 : 24576;  // values per accumulate pass (96 KB)
     static constexpr size_t lds = sizeof(V) * VCAP + sizeof(uint2) * WORDS;
 };
 
@@ -491,7 +496,9 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
 {
     using R = RankWin<V>;
     constexpr int NU = kLongBS / kWave;
-    constexpr int WPT = R::WORDS / kLongBS;  // bitmap words per thread in the scans
+    constexpr int WPT = R::WORDS / kLongBS;  // bitmap words per thread
+    constexpr int WPW = R::WORDS / NU;       // bitmap words per wave in the rank scan
+    constexpr int RPW = WPW / kWave;
     static_assert(R::WORDS % kLongBS == 0, "words per thread");
     extern __shared__ unsigned long long smem_u64[];
     V *vals = reinterpret_cast<V *>(smem_u64);
@@ -502,14 +509,14 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     __shared__ BlockStage<kLongBS, V> s_bst;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
-    const int wb0 = tid * WPT;
+    const int lane = lane_id();
     int *const cursor = ls.cursor;
     const long long span = ls.span;
     int *const cs[2] = {cursor, cursor + span};
     int *const cz[2] = {cursor + 2 * span, cursor + 3 * span};
     const int cstride = ls.nwin + 1;
 
-    for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
+    for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
     __syncthreads();
 
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
@@ -555,15 +562,30 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             nxt = unit_min<kWave>(nxt);
             if (lane_id() == 0) s_min[wave] = nxt;
             __syncthreads();
-            // 2. rank
-            int local = 0;
-            for (int i = 0; i < WPT; i++) local += __popc(br[wb0 + i].x);
-            int tot;
-            const int first_rank = block_excl_scan<kLongBS>(local, s_scan, tot);
-            int run = first_rank;
-            for (int i = 0; i < WPT; i++) {
-                br[wb0 + i].y = static_cast<unsigned>(run);
-                run += __popc(br[wb0 + i].x);
+            // 2. rank: wave w owns words [w * WPW, (w + 1) * WPW) and its
+            // lanes take them 64 consecutive words at a time (conflict-free
+            // LDS reads); ranks come from DPP scans of the words' bit counts.
+            int cnt[RPW], inc[RPW], rsum[RPW];
+            int wtot = 0;
+#pragma unroll
+            for (int i = 0; i < RPW; i++) {
+                cnt[i] = __popc(br[wave * WPW + i * kWave + lane].x);
+                inc[i] = wave_incl_scan(cnt[i]);
+                rsum[i] = __builtin_amdgcn_readlane(inc[i], kWave - 1);
+                wtot += rsum[i];
+            }
+            if (lane == 0) s_scan[wave] = wtot;
+            __syncthreads();
+            int tot = 0, run = 0;
+            for (int w = 0; w < NU; w++) {
+                const int t = s_scan[w];
+                run += w < wave ? t : 0;
+                tot += t;
+            }
+#pragma unroll
+            for (int i = 0; i < RPW; i++) {
+                br[wave * WPW + i * kWave + lane].y = static_cast<unsigned>(run + inc[i] - cnt[i]);
+                run += rsum[i];
             }
             nxt = INT_MAX;
             for (int w = 0; w < NU; w++) nxt = min(nxt, s_min[w]);
@@ -574,16 +596,16 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 const int r1 = min(tot, r0 + R::VCAP);
                 int cb = wend;
                 if (r1 < tot) {
-                    // The thread whose words hold rank r1 finds its column.
-                    if (first_rank <= r1 && r1 < run) {
-                        for (int i = 0; i < WPT; i++) {
-                            const unsigned bits = br[wb0 + i].x;
-                            const int base = static_cast<int>(br[wb0 + i].y);
-                            if (r1 >= base && r1 < base + __popc(bits)) {
-                                unsigned b = bits;
-                                for (int skip = r1 - base; skip > 0; skip--) b &= b - 1u;
-                                s_cut = w0 + (wb0 + i) * 32 + (__ffs(b) - 1);
-                            }
+                    // The lane whose word holds rank r1 finds its column.
+#pragma unroll
+                    for (int i = 0; i < RPW; i++) {
+                        const int wd = wave * WPW + i * kWave + lane;
+                        const uint2 ew = br[wd];
+                        const int base = static_cast<int>(ew.y);
+                        if (r1 >= base && r1 < base + __popc(ew.x)) {
+                            unsigned b = ew.x;
+                            for (int skip = r1 - base; skip > 0; skip--) b &= b - 1u;
+                            s_cut = w0 + wd * 32 + (__ffs(b) - 1);
                         }
                     }
                     __syncthreads();
@@ -656,7 +678,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 ca = cb;
                 sub++;
             }
-            for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
+            for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
             __syncthreads();
             out += tot;
             first = false;

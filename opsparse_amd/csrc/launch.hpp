@@ -111,7 +111,7 @@ size_t symbolic_long_lds();
 constexpr int kSpillBin = 5;
 
 // Column window of the long-row bitmap kernels (numeric and backward).
-constexpr int kRankWindow = 1 << 18;
+constexpr int kRankWindow = 1 << 17;
 // Largest window-cut table built (bytes); beyond it the long-row kernels
 // locate window boundaries by galloping in each B row instead.
 constexpr long long kMaxCutBytes = 1ll << 30;

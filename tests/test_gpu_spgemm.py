@@ -181,9 +181,10 @@ def test_output_cache_reuse_and_limit(gpu):
 
 
 def test_long_rows_many_rank_subwindows(gpu):
-    """Rows of ~90K entries inside one 2^18-column bitmap window: the
-    accumulate pass runs over many value sub-windows (8192 FP64 / 16384 FP32
-    entries each), cut at the column of each rank boundary."""
+    """Rows of ~90K entries over two 2^17-column bitmap windows: the
+    accumulate pass runs over several value sub-windows per window (12288
+    FP64 / 24576 FP32 entries each), cut at the column of each rank
+    boundary."""
     rng = np.random.default_rng(12)
     N = 200000
     A = random_csr(6, 1000, np.r_[rng.integers(350, 450, 4), rng.integers(1, 3, 2)], rng)
@@ -194,10 +195,11 @@ def test_long_rows_many_rank_subwindows(gpu):
 
 
 def test_long_rows_multi_window_cut_table_and_gallop(gpu, monkeypatch):
-    """Long rows over 6 column windows of 2^18 with windows 1, 3 and 4
-    empty: the mark pass skips them, window 2 needs several value
-    sub-windows.  Run with the window-cut table of B (default) and with the
-    table turned off (window boundaries found by galloping in each B row)."""
+    """Long rows whose columns lie in blocks 0, 2 and 5 of six 2^18-column
+    blocks: the bitmap windows (2^17 columns) in between are empty and
+    skipped, and the filled windows need several value sub-windows.  Run with
+    the window-cut table of B (default) and with the table turned off
+    (window boundaries found by galloping in each B row)."""
     rng = np.random.default_rng(14)
     W = 1 << 18
     N = 5 * W + 7
