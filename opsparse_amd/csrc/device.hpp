@@ -671,11 +671,20 @@ __device__ __forceinline__ void block_walk(const BlockStage<BS, V> &st, int n, i
                 if (sb < s_end) {
                     const int qc = qa + 1 + lane;
                     const int pos = qc <= n ? st.excl[qc] - sb : kWave + 1;
-                    const unsigned lo32 = pos > 0 && pos < 32 ? 1u << pos : 0u;
-                    const unsigned hi32 = pos >= 32 && pos < kWave ? 1u << (pos - 32) : 0u;
-                    const unsigned long long m = (static_cast<unsigned long long>(wave_or(hi32)) << 32) | wave_or(lo32);
-                    const int qo = qa + __popcll(m & below);
-                    qa += __popcll(m) + (__ballot(pos == kWave) != 0ull ? 1 : 0);
+                    // Lane 0's candidate is the next range: when it starts at
+                    // or after the step's end, the whole step lies in range qa.
+                    const int pos0 = __builtin_amdgcn_readfirstlane(pos);
+                    int qo = qa;
+                    if (pos0 >= kWave) {
+                        qa += pos0 == kWave ? 1 : 0;
+                    } else {
+                        const unsigned lo32 = pos > 0 && pos < 32 ? 1u << pos : 0u;
+                        const unsigned hi32 = pos >= 32 && pos < kWave ? 1u << (pos - 32) : 0u;
+                        const unsigned long long m =
+                            (static_cast<unsigned long long>(wave_or(hi32)) << 32) | wave_or(lo32);
+                        qo = qa + __popcll(m & below);
+                        qa += __popcll(m) + (__ballot(pos == kWave) != 0ull ? 1 : 0);
+                    }
                     const int p = sb + lane;
                     ok[s] = p < s_end;
                     if (ok[s]) {

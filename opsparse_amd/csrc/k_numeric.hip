@@ -128,6 +128,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
 // y: number of set bits in the words before), so the rank of a column is
 // one 64-bit LDS read.
 // Step 1 consumer: set the product's column bit.
+// The OR returns nothing, so no product waits on an LDS result.
 template <typename V>
 struct RankMark {
     const int *__restrict__ bcol;
@@ -135,12 +136,10 @@ struct RankMark {
     int w0;
     using Item = int;
     __device__ __forceinline__ int load(int k) const { return bcol[k]; }
-    __device__ __forceinline__ unsigned prep(int col) const { return br[(col - w0) >> 5].x; }
-    __device__ __forceinline__ void commit(int col, unsigned word, V, int)
+    __device__ __forceinline__ void apply(int col, V, int)
     {
         const int c = col - w0;
-        const unsigned bit = 1u << (c & 31);
-        if (!(word & bit)) atomicOr(&br[c >> 5].x, bit);
+        atomicOr(&br[c >> 5].x, 1u << (c & 31));
     }
 };
 

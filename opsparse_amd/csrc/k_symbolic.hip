@@ -38,22 +38,18 @@ struct SymInsert {
     }
 };
 
+// Sets the column's bit with an OR that returns nothing (no product waits on
+// an LDS result); the window's count is the bitmap's popcount afterwards.
 struct BitmapInsert {
     const int *__restrict__ bcol;
     unsigned *bm;
     int w0;
-    int cnt = 0;
     using Item = int;
     __device__ __forceinline__ int load(int k) const { return bcol[k]; }
     __device__ __forceinline__ void apply(int col, double, int)
     {
         const int c = col - w0;
-        const unsigned bit = 1u << (c & 31);
-        // Popular columns are set early; a plain read skips the atomic (and
-        // its same-address serialisation) for them.
-        if (bm[c >> 5] & bit) return;
-        const unsigned old = atomicOr(&bm[c >> 5], bit);
-        cnt += (old & bit) ? 0 : 1;
+        atomicOr(&bm[c >> 5], 1u << (c & 31));
     }
 };
 
@@ -150,6 +146,9 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
     __shared__ int s_red[kLongBS / kWave];
     __shared__ BlockStage<kLongBS, double> s_bst;
     const int tid = threadIdx.x;
+    // The bitmap is cleared once here and then by each window's count pass.
+    for (int i = tid; i < (wcols >> 5); i += kLongBS) bm[i] = 0u;
+    __syncthreads();
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
         const int r = rows ? rows[ridx] : ridx;
         const int a_beg = arpt[r], a_end = arpt[r + 1];
@@ -157,14 +156,17 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
         for (int w0 = 0; w0 < ncols; w0 += wcols) {
             const int wend = min(ncols, w0 + wcols);
             const int nwords = (wend - w0 + 31) >> 5;
-            for (int i = tid; i < nwords; i += kLongBS) bm[i] = 0u;
-            __syncthreads();
             const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols};
             BitmapInsert op{bcol, bm, w0};
             int sn, stot;
             block_for_each_product<kLongBS, false, double>(a_beg, a_end, acol, static_cast<const double *>(nullptr),
                                                            range, op, s_bst, sn, stot);
-            cnt += op.cnt;
+            __syncthreads();
+            for (int i = tid; i < nwords; i += kLongBS) {
+                cnt += __popc(bm[i]);
+                bm[i] = 0u;
+            }
+            __syncthreads();
         }
         const int tot = block_sum<kLongBS>(cnt, s_red);
         if (tid == 0) row_nnz[r] = tot;
