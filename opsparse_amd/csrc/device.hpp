@@ -544,6 +544,20 @@ __device__ __forceinline__ unsigned wave_or(unsigned v)
     return static_cast<unsigned>(__builtin_amdgcn_readlane(x, kWave - 1));
 }
 
+// Lane 0's value in every lane, held in scalar registers.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float wave_uniform(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double wave_uniform(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b));
+    const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
 // Inclusive prefix sum over the 64 lanes of the wave: Hillis-Steele steps
 // by DPP row shifts inside each row of 16 lanes, then the row broadcasts
 // add row 0's total to row 1 and row 2's to row 3, and rows 0-1's total to
@@ -577,28 +591,19 @@ struct BlockStage {
     int wsum[2][NW];
 };
 
-// Stages A-nonzeros [c, min(c + BS, a_end)) (one per thread) into `st`
-// (aval may be nullptr: the staged values are then 0):
-// each thread computes its own A-nonzero's range, so a range is computed
-// once per batch whatever the number of waves.  Returns the number of
-// staged (non-empty) ranges; `total` gets the batch's product count.
-// Every thread of the block calls it; it ends with a barrier.  The
-// caller's previous walk of `st` must be complete (a barrier) before.
-template <int BS, typename V, typename Range>
-__device__ __forceinline__ int block_stage(int c, int a_end, const int *__restrict__ acol, const V *__restrict__ aval,
-                                           const Range &range, BlockStage<BS, V> &st, int &total)
+// Stages one batch from per-thread ranges (thread t holds A-nonzero e with
+// B range [bs, bs + len) and A value av; len = 0 for none) into `st`: the
+// non-empty ranges in thread order.  Returns their number; `total` gets the
+// batch's product count.  Every thread of the block calls it; it ends with a
+// barrier.  The caller's previous walk of `st` must be complete (a barrier)
+// before.
+template <int BS, typename V>
+__device__ __forceinline__ int block_stage_ranges(int e, int bs, int len, V av, BlockStage<BS, V> &st, int &total)
 {
     constexpr int NW = BS / kWave;
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wave = tid / kWave;
-    const int e = c + tid;
-    int bs = 0, len = 0;
-    V av = V(0);
-    if (e < a_end) {
-        range(e, acol[e], bs, len);
-        if (aval) av = aval[e];
-    }
     const int f = len > 0 ? 1 : 0;
     const int il = wave_incl_scan(len);
     const int iff = wave_incl_scan(f);
@@ -628,6 +633,24 @@ __device__ __forceinline__ int block_stage(int c, int a_end, const int *__restri
     __syncthreads();
     total = tl;
     return tf;
+}
+
+// Stages A-nonzeros [c, min(c + BS, a_end)) (one per thread) into `st`
+// (aval may be nullptr: the staged values are then 0): each thread computes
+// its own A-nonzero's range, so a range is computed once per batch whatever
+// the number of waves.
+template <int BS, typename V, typename Range>
+__device__ __forceinline__ int block_stage(int c, int a_end, const int *__restrict__ acol, const V *__restrict__ aval,
+                                           const Range &range, BlockStage<BS, V> &st, int &total)
+{
+    const int e = c + static_cast<int>(threadIdx.x);
+    int bs = 0, len = 0;
+    V av = V(0);
+    if (e < a_end) {
+        range(e, acol[e], bs, len);
+        if (aval) av = aval[e];
+    }
+    return block_stage_ranges<BS, V>(e, bs, len, av, st, total);
 }
 
 // Walks every product of a staged batch with all waves of the block: wave
@@ -663,35 +686,56 @@ __device__ __forceinline__ void block_walk(const BlockStage<BS, V> &st, int n, i
         cnt = __popcll(__ballot(lane < g1 && q < n && st.excl[q] <= s_beg));
         int qa = lo + cnt - 1;
         const unsigned long long below = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
+        // Wave-uniform copy of range qa: where the next range starts, and
+        // qa's B offset, A-nonzero and A value.  A step that ends at or
+        // before q_next lies in range qa and needs no LDS read at all.
+        int q_next = 0, q_off = 0, q_e = 0;
+        V q_av = V(0);
+        auto load_q = [&]() {
+            q_next = __builtin_amdgcn_readfirstlane(qa + 1 <= n ? st.excl[qa + 1] : INT_MAX);
+            if (qa < n) {
+                const typename BlockStage<BS, V>::Ent en = st.ent[qa];
+                q_off = __builtin_amdgcn_readfirstlane(en.off);
+                q_e = __builtin_amdgcn_readfirstlane(en.e);
+                q_av = wave_uniform(en.av);
+            }
+        };
+        load_q();
         auto gather = [&](int b0, int (&bi)[NP], V (&aa)[NP], int (&qq)[NP], bool (&ok)[NP]) {
 #pragma unroll
             for (int s = 0; s < NP; s++) {
                 const int sb = b0 + s * kWave;
                 ok[s] = false;
                 if (sb < s_end) {
-                    const int qc = qa + 1 + lane;
-                    const int pos = qc <= n ? st.excl[qc] - sb : kWave + 1;
-                    // Lane 0's candidate is the next range: when it starts at
-                    // or after the step's end, the whole step lies in range qa.
-                    const int pos0 = __builtin_amdgcn_readfirstlane(pos);
-                    int qo = qa;
-                    if (pos0 >= kWave) {
-                        qa += pos0 == kWave ? 1 : 0;
+                    const int p = sb + lane;
+                    ok[s] = p < s_end;
+                    if (sb + kWave <= q_next) {
+                        bi[s] = p + q_off;
+                        aa[s] = q_av;
+                        qq[s] = q_e;
+                        if (sb + kWave == q_next) {
+                            qa++;
+                            load_q();
+                        }
                     } else {
+                        // Lane l reads the start of range qa+1+l; the starts
+                        // inside the step form a mask, and a lane's owner is qa
+                        // plus the starts at or before it.
+                        const int qc = qa + 1 + lane;
+                        const int pos = qc <= n ? st.excl[qc] - sb : kWave + 1;
                         const unsigned lo32 = pos > 0 && pos < 32 ? 1u << pos : 0u;
                         const unsigned hi32 = pos >= 32 && pos < kWave ? 1u << (pos - 32) : 0u;
                         const unsigned long long m =
                             (static_cast<unsigned long long>(wave_or(hi32)) << 32) | wave_or(lo32);
-                        qo = qa + __popcll(m & below);
+                        const int qo = qa + __popcll(m & below);
                         qa += __popcll(m) + (__ballot(pos == kWave) != 0ull ? 1 : 0);
-                    }
-                    const int p = sb + lane;
-                    ok[s] = p < s_end;
-                    if (ok[s]) {
-                        const typename BlockStage<BS, V>::Ent en = st.ent[qo];
-                        bi[s] = p + en.off;
-                        aa[s] = en.av;
-                        qq[s] = en.e;
+                        if (ok[s]) {
+                            const typename BlockStage<BS, V>::Ent en = st.ent[qo];
+                            bi[s] = p + en.off;
+                            aa[s] = en.av;
+                            qq[s] = en.e;
+                        }
+                        load_q();
                     }
                 }
             }
