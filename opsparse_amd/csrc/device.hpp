@@ -202,11 +202,36 @@ __device__ __forceinline__ void load4_vals(const double *__restrict__ p, double 
 }
 __device__ __forceinline__ bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
+// A prep consumer may split commit further into
+//   Post stage(const Item &, const Slot &, V a, int q, bool ok) const;  // arithmetic only
+//   void post(const Post &);                                           // the updates
+// stage runs for every product of a step (also for !ok ones, whose Slot
+// is unset and whose Post must then do nothing), so the step's LDS reads
+// are waited for once and its updates issue back to back; with commit,
+// the branch around each product makes the compiler wait for all earlier
+// LDS operations (the previous products' atomics included) before each one.
+template <typename Op, typename = void>
+struct has_post : std::false_type {
+};
+template <typename Op>
+struct has_post<Op, std::void_t<decltype(&Op::post)>> : std::true_type {
+};
+
 template <int NP, typename Op, typename V>
 __device__ __forceinline__ void apply_batch(Op &op, const typename Op::Item (&it)[NP], const V (&a)[NP],
                                             const int (&q)[NP], const bool (&ok)[NP])
 {
-    if constexpr (has_prep<Op>::value) {
+    if constexpr (has_post<Op>::value) {
+        decltype(op.prep(it[0])) sl[NP];
+#pragma unroll
+        for (int n = 0; n < NP; n++)
+            if (ok[n]) sl[n] = op.prep(it[n]);
+        decltype(op.stage(it[0], sl[0], a[0], q[0], ok[0])) ps[NP];
+#pragma unroll
+        for (int n = 0; n < NP; n++) ps[n] = op.stage(it[n], sl[n], a[n], q[n], ok[n]);
+#pragma unroll
+        for (int n = 0; n < NP; n++) op.post(ps[n]);
+    } else if constexpr (has_prep<Op>::value) {
         decltype(op.prep(it[0])) sl[NP];
 #pragma unroll
         for (int n = 0; n < NP; n++)
@@ -544,6 +569,17 @@ __device__ __forceinline__ unsigned wave_or(unsigned v)
     return static_cast<unsigned>(__builtin_amdgcn_readlane(x, kWave - 1));
 }
 
+// Waits until every global store (and load) of this wave has completed.
+// The compiler's wait counting treats loads and stores as one counter that
+// may complete out of order when both are in flight, and then waits for
+// everything before each use of a loaded value; a kernel that writes
+// output and then walks products calls this between the two, once, so
+// the walk's loads are waited for one buffer at a time.
+__device__ __forceinline__ void drain_vmem()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unconstrained
+}
+
 // Lane 0's value in every lane, held in scalar registers.
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float wave_uniform(float v)
@@ -701,16 +737,22 @@ __device__ __forceinline__ void block_walk(const BlockStage<BS, V> &st, int n, i
             }
         };
         load_q();
+        // Steps past the slice, and lanes past its end, get ok = false and
+        // B index 0 (a valid entry: the batch has products), so every load
+        // below is issued unconditionally.
         auto gather = [&](int b0, int (&bi)[NP], V (&aa)[NP], int (&qq)[NP], bool (&ok)[NP]) {
 #pragma unroll
             for (int s = 0; s < NP; s++) {
                 const int sb = b0 + s * kWave;
                 ok[s] = false;
+                bi[s] = 0;
+                aa[s] = V(0);
+                qq[s] = 0;
                 if (sb < s_end) {
                     const int p = sb + lane;
                     ok[s] = p < s_end;
                     if (sb + kWave <= q_next) {
-                        bi[s] = p + q_off;
+                        if (ok[s]) bi[s] = p + q_off;
                         aa[s] = q_av;
                         qq[s] = q_e;
                         if (sb + kWave == q_next) {
@@ -740,44 +782,37 @@ __device__ __forceinline__ void block_walk(const BlockStage<BS, V> &st, int n, i
                 }
             }
         };
+        // Two buffers of NP steps in ping-pong, so no loaded value is copied
+        // (a copy would wait for the load) and the loads of one buffer are in
+        // flight while the other's updates run.  All loads are unconditional,
+        // which lets the compiler wait for exactly the buffer it consumes.
+        // Every load is issued inside the loop (none before it), so the
+        // loop's entry holds no load in flight and the compiler's waits at
+        // the loop head stay exact.  Buffer B starts empty (ok = false).
         const int stride = NP * kWave;
-        int base = s_beg;
-        int bb[NP], qq[NP];
-        V aa[NP];
-        bool ok[NP];
-        typename Op::Item it[NP];
-        gather(base, bb, aa, qq, ok);
+        int bbA[NP], qqA[NP], bbB[NP], qqB[NP];
+        V aaA[NP], aaB[NP];
+        bool okA[NP], okB[NP];
+        typename Op::Item itA[NP], itB[NP];
 #pragma unroll
-        for (int s = 0; s < NP; s++)
-            if (ok[s]) it[s] = op.load(bb[s]);
-        while (true) {
-            const int nb = base + stride;
-            const bool more = nb < s_end;
-            int bb2[NP], qq2[NP];
-            V aa2[NP];
-            bool ok2[NP];
-            typename Op::Item it2[NP];
-            if (more) {
-                gather(nb, bb2, aa2, qq2, ok2);
-#pragma unroll
-                for (int s = 0; s < NP; s++)
-                    if (ok2[s]) it2[s] = op.load(bb2[s]);
-            }
-            if constexpr (!WITH_A) {
-#pragma unroll
-                for (int s = 0; s < NP; s++) aa[s] = V(0);
-            }
-            apply_batch<NP>(op, it, aa, qq, ok);
-            if (!more) break;
-            base = nb;
-#pragma unroll
-            for (int s = 0; s < NP; s++) {
-                it[s] = it2[s];
-                aa[s] = aa2[s];
-                qq[s] = qq2[s];
-                ok[s] = ok2[s];
-            }
+        for (int s = 0; s < NP; s++) {
+            okB[s] = false;
+            qqB[s] = 0;
+            aaB[s] = V(0);
+            itB[s] = typename Op::Item{};
         }
+        for (int base = s_beg;; base += 2 * stride) {
+            gather(base, bbA, aaA, qqA, okA);
+#pragma unroll
+            for (int s = 0; s < NP; s++) itA[s] = op.load(bbA[s]);
+            apply_batch<NP>(op, itB, aaB, qqB, okB);
+            gather(base + stride, bbB, aaB, qqB, okB);
+#pragma unroll
+            for (int s = 0; s < NP; s++) itB[s] = op.load(bbB[s]);
+            apply_batch<NP>(op, itA, aaA, qqA, okA);
+            if (base + 2 * stride >= s_end) break;
+        }
+        apply_batch<NP>(op, itB, aaB, qqB, okB);
     }
     __syncthreads();
 }

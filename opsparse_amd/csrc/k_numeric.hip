@@ -158,13 +158,21 @@ struct RankAcc {
         int c;
         V b;
     };
+    struct Post {
+        unsigned at;  // >= nr: nothing to add
+        V v;
+    };
     __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
     __device__ __forceinline__ uint2 prep(const Item &it) const { return br[(it.c - w0) >> 5]; }
-    __device__ __forceinline__ void commit(const Item &it, const uint2 &e, V a, int)
+    __device__ __forceinline__ Post stage(const Item &it, const uint2 &e, V a, int, bool ok) const
     {
         const int c = it.c - w0;
         const unsigned at = e.y + __popc(e.x & ((1u << (c & 31)) - 1u)) - static_cast<unsigned>(r0);
-        if (at < nr) atomicAdd(vals + at, a * it.b);
+        return Post{ok ? at : nr, a * it.b};
+    }
+    __device__ __forceinline__ void post(const Post &p)
+    {
+        if (p.at < nr) atomicAdd(vals + p.at, p.v);
     }
 };
 
@@ -511,8 +519,10 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     const int lane = lane_id();
     int *const cursor = ls.cursor;
     const long long span = ls.span;
-    int *const cs[2] = {cursor, cursor + span};
-    int *const cz[2] = {cursor + 2 * span, cursor + 3 * span};
+    // Cursor arrays picked by parity with selects (an indexed array of
+    // pointers would live in scratch memory).
+    auto cs = [&](int i) { return cursor + (i & 1) * span; };
+    auto cz = [&](int i) { return cursor + (2 + (i & 1)) * span; };
     const int cstride = ls.nwin + 1;
 
     for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
@@ -542,6 +552,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             const int wend = (ncols - w0 <= R::WB) ? ncols : w0 + R::WB;
             const bool last = wend >= ncols;
             // 1. mark
+            drain_vmem();
             int nxt = INT_MAX;
             int sn = 0, stot = 0;
             bool one = false;
@@ -553,7 +564,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     one = block_for_each_product<kLongBS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                     stot);
                 } else {
-                    const CursorRange range{brpt, bcol, cs[p], cs[p ^ 1], wend, first, last, &nxt};
+                    const CursorRange range{brpt, bcol, cs(p), cs(p ^ 1), wend, first, last, &nxt};
                     one = block_for_each_product<kLongBS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                     stot);
                 }
@@ -612,6 +623,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 }
                 for (int i = tid; i < r1 - r0; i += kLongBS) vals[i] = V(0);
                 __syncthreads();
+                drain_vmem();
                 RankAcc<V> op{bcol, bval, br, vals, w0, r0, static_cast<unsigned>(r1 - r0)};
                 if (tot <= 2 * R::VCAP) {
                     // At most two sub-windows: walk the window's products over
@@ -625,20 +637,20 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                         block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     } else {
-                        const StoredRange range{brpt, cs[p], cs[p ^ 1], first};
+                        const StoredRange range{brpt, cs(p), cs(p ^ 1), first};
                         block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     }
                 } else {
                     int n2, t2;
                     if (ls.wcut) {
-                        const CutCursorRange range{ls.wcut,    bcol, cstride,    wi, cz[(sub - 1) & 1], cz[sub & 1],
+                        const CutCursorRange range{ls.wcut,    bcol, cstride,    wi, cz(sub - 1), cz(sub),
                                                    cb,         sub == 0, cb == wend};
                         block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     } else {
                         int unused = INT_MAX;
-                        const int *cin = sub == 0 ? cs[p] : cz[(sub - 1) & 1];
-                        const CursorRange range{brpt,       bcol, cin, cz[sub & 1], cb, first && sub == 0,
+                        const int *cin = sub == 0 ? cs(p) : cz(sub - 1);
+                        const CursorRange range{brpt,       bcol, cin, cz(sub), cb, first && sub == 0,
                                                 last && cb == wend, &unused};
                         block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     }

@@ -157,6 +157,7 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
             const int wend = min(ncols, w0 + wcols);
             const int nwords = (wend - w0 + 31) >> 5;
             const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols};
+            drain_vmem();
             BitmapInsert op{bcol, bm, w0};
             int sn, stot;
             block_for_each_product<kLongBS, false, double>(a_beg, a_end, acol, static_cast<const double *>(nullptr),
