@@ -84,6 +84,27 @@ def test_stencil_small(gpu):
     run(A, A)
 
 
+def test_short_bins_narrow_and_wide_rows(gpu):
+    """Rows with few products, alternating between a narrow column span
+    (banded: the symbolic kernels count them in a bitmap) and a wide one
+    (hashed), in the same launches of bins 0-4."""
+    rng = np.random.default_rng(15)
+    N = 20000
+    b_rows = []
+    for k in range(N):
+        c = np.arange(max(0, k - 40), min(N, k + 41))
+        b_rows.append(np.sort(rng.choice(c, size=min(c.size, int(rng.integers(2, 13))), replace=False)))
+    B = csr_from_rows(N, N, b_rows, rng)
+    a_rows = []
+    for i in range(N):
+        n = int(rng.integers(1, 31))
+        pool = np.arange(max(0, i - 50), min(N, i + 51)) if i % 2 == 0 else np.arange(N)
+        a_rows.append(np.sort(rng.choice(pool, size=min(n, pool.size), replace=False)))
+    A = csr_from_rows(N, N, a_rows, rng)
+    run(A, B)
+    run(A, B, np.float32)
+
+
 def test_every_bin_rmat(gpu):
     A = G.rmat(13, 16, 7)
     t = G.compute_flop_host(A, A)
