@@ -278,7 +278,7 @@ def run_batch(args):
         Ah, Bh = b.A.astype(npdt), b.B.astype(npdt)
         dA = DeviceCSR.from_host(Ah, dt)
         dB = DeviceCSR.from_host(Bh, dt)
-        tune = autotune.autotune(dA, dB, ws=ws, wide=True, reps=1)
+        tune = autotune.autotune(dA, dB, ws=ws, wide=True, reps=3)
         log(f"batch {tag}: tuning {tune['tuning']} (cached: {tune['from_cache']})")
         elapsed, tims, c_last = timed_calls(dA, dB, ws, args.steps, args.warmup, True)
         t_fused = elapsed / args.steps

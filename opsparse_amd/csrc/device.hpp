@@ -163,45 +163,6 @@ template <typename Op>
 struct has_prep<Op, std::void_t<decltype(&Op::prep)>> : std::true_type {
 };
 
-// A consumer may also define
-//   void load4(int b_index, Item (&it)[4]) const;   // b_index % 4 == 0
-//   bool vec_ok() const;                            // B arrays 16-byte aligned
-// reading four consecutive B entries with 16-byte loads; the grouped walk
-// then takes long B rows four entries per lane per step.
-template <typename Op, typename = void>
-struct has_load4 : std::false_type {
-};
-template <typename Op>
-struct has_load4<Op, std::void_t<decltype(&Op::load4)>> : std::true_type {
-};
-
-__device__ __forceinline__ void load4_cols(const int *__restrict__ p, int (&c)[4])
-{
-    const int4 v = *reinterpret_cast<const int4 *>(p);
-    c[0] = v.x;
-    c[1] = v.y;
-    c[2] = v.z;
-    c[3] = v.w;
-}
-__device__ __forceinline__ void load4_vals(const float *__restrict__ p, float (&b)[4])
-{
-    const float4 v = *reinterpret_cast<const float4 *>(p);
-    b[0] = v.x;
-    b[1] = v.y;
-    b[2] = v.z;
-    b[3] = v.w;
-}
-__device__ __forceinline__ void load4_vals(const double *__restrict__ p, double (&b)[4])
-{
-    const double2 v0 = *reinterpret_cast<const double2 *>(p);
-    const double2 v1 = *reinterpret_cast<const double2 *>(p + 2);
-    b[0] = v0.x;
-    b[1] = v0.y;
-    b[2] = v1.x;
-    b[3] = v1.y;
-}
-__device__ __forceinline__ bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
-
 // A prep consumer may split commit further into
 //   Post stage(const Item &, const Slot &, V a, int q, bool ok) const;  // arithmetic only
 //   void post(const Post &);                                           // the updates
@@ -247,7 +208,7 @@ __device__ __forceinline__ void apply_batch(Op &op, const typename Op::Item (&it
 }
 
 // Grouped walk of one loaded chunk: GS lanes per A-nonzero, each group
-// strides over its B row two entries at a time.  No per-product cross-lane
+// strides over its B row four entries per lane at a time.  No per-product cross-lane
 // search; used when the chunk's B rows have similar lengths.
 template <int U, int GS, bool WITH_A, typename V, bool SLICE, typename Op>
 __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op &op, int part, int nparts)
@@ -264,8 +225,11 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
     // every B row (part p takes steps p, p+nparts, ... of each row).
     // Slicing keeps all parts busy on rows with few A-nonzeros and long B
     // rows; group distribution suits many A-nonzeros with short B rows.
-    const int t0 = SLICE ? sub + part * 2 * GS : sub;
-    const int tstep = SLICE ? nparts * 2 * GS : 2 * GS;
+    // Each lane takes NPG entries of its group's B row per step (their loads
+    // are issued together).
+    constexpr int NPG = 4;
+    const int t0 = SLICE ? sub + part * NPG * GS : sub;
+    const int tstep = SLICE ? nparts * NPG * GS : NPG * GS;
     const int qstart = SLICE ? 0 : part * PER;
     const int qstep = SLICE ? PER : nparts * PER;
     for (int q0 = qstart; q0 < cnt; q0 += qstep) {
@@ -278,42 +242,17 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
         const int ql = q < cnt ? lq : 0;
         V qa = V(0);
         if constexpr (WITH_A) qa = bperm(av, src);
-        const V a2[2] = {qa, qa};
-        const int q2[2] = {q, q};
-        if constexpr (has_load4<Op>::value && GS >= 16) {
-            if (op.vec_ok()) {
-                // Four consecutive entries per lane per step from the first
-                // 4-aligned entry; the unaligned head and the tail (at most
-                // three entries each) go to the part that takes step 0.
-                const int h_end = min(qb + ql, (qb + 3) & ~3);
-                const int nvec = (qb + ql - h_end) >> 2;
-                const int b_end = h_end + 4 * nvec;
-                if (!SLICE || part == 0) {
-                    typename Op::Item it[2];
-                    const bool ok[2] = {sub < h_end - qb, sub < qb + ql - b_end};
-                    if (ok[0]) it[0] = op.load(qb + sub);
-                    if (ok[1]) it[1] = op.load(b_end + sub);
-                    apply_batch<2>(op, it, a2, q2, ok);
-                }
-                const int v0 = SLICE ? sub + part * GS : sub;
-                const int vstep = SLICE ? nparts * GS : GS;
-                const V a4[4] = {qa, qa, qa, qa};
-                const int q4[4] = {q, q, q, q};
-                const bool ok[4] = {true, true, true, true};
-                for (int v = v0; v < nvec; v += vstep) {
-                    typename Op::Item it[4];
-                    op.load4(h_end + 4 * v, it);
-                    apply_batch<4>(op, it, a4, q4, ok);
-                }
-                continue;
-            }
-        }
+        const V aa[NPG] = {qa, qa, qa, qa};
+        const int qq[NPG] = {q, q, q, q};
         for (int t = t0; t < ql; t += tstep) {
-            typename Op::Item it[2];
-            const bool ok[2] = {true, t + GS < ql};
-            it[0] = op.load(qb + t);
-            if (ok[1]) it[1] = op.load(qb + t + GS);
-            apply_batch<2>(op, it, a2, q2, ok);
+            typename Op::Item it[NPG];
+            bool ok[NPG];
+#pragma unroll
+            for (int n = 0; n < NPG; n++) {
+                ok[n] = t + n * GS < ql;
+                if (ok[n]) it[n] = op.load(qb + t + n * GS);
+            }
+            apply_batch<NPG>(op, it, aa, qq, ok);
         }
     }
 }
