@@ -204,6 +204,9 @@ typedef struct ops_tuning {
     int32_t sym_bound[OPS_NUM_BINS - 1]; /* inclusive upper bound of row products per symbolic bin */
     int32_t num_bound[OPS_NUM_BINS - 1]; /* inclusive upper bound of row nnz per numeric bin */
     int32_t short_hash;                  /* bins 0-4: 0 = linear (col * 107 mod T), 1 = Fibonacci */
+    int32_t sym_bitmap;                  /* symbolic bins 0-4: 1 = a row whose products span fewer than
+                                            32*T columns sets bits in its T-word table and counts them
+                                            instead of hashing; 0 = always hash */
 } ops_tuning;
 
 int ops_tuning_default(ops_tuning *t);

@@ -58,10 +58,11 @@ class OpsTimings(C.Structure):
 
 class OpsTuning(C.Structure):
     _fields_ = [("sym_bound", C.c_int32 * (NUM_BINS - 1)), ("num_bound", C.c_int32 * (NUM_BINS - 1)),
-                ("short_hash", C.c_int32)]
+                ("short_hash", C.c_int32), ("sym_bitmap", C.c_int32)]
 
     def as_dict(self):
-        return {"sym_bound": list(self.sym_bound), "num_bound": list(self.num_bound), "short_hash": self.short_hash}
+        return {"sym_bound": list(self.sym_bound), "num_bound": list(self.num_bound), "short_hash": self.short_hash,
+                "sym_bitmap": self.sym_bitmap}
 
     @staticmethod
     def from_dict(d):
@@ -71,6 +72,7 @@ class OpsTuning(C.Structure):
         for i, v in enumerate(d["num_bound"]):
             t.num_bound[i] = v
         t.short_hash = d["short_hash"]
+        t.sym_bitmap = d.get("sym_bitmap", 0)
         return t
 
 

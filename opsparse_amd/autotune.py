@@ -12,7 +12,8 @@ Candidate space (every candidate respects the kernels' capacities):
   * short-row hash: linear (col*107) or Fibonacci;
   * symbolic bounds: default, or each bound halved (rows move to wider
     kernels with larger tables earlier);
-  * numeric bounds: default, or each bound halved.
+  * numeric bounds: default, or each bound halved;
+  * symbolic bitmap for rows with a narrow column span: off or on.
 """
 from __future__ import annotations
 
@@ -42,15 +43,16 @@ def candidates() -> list[dict]:
     for short_hash in (0, 1):
         for sym_half in (False, True):
             for num_half in (False, True):
-                t = {"sym_bound": list(base["sym_bound"]), "num_bound": list(base["num_bound"]),
-                     "short_hash": short_hash}
-                if sym_half:
-                    t["sym_bound"] = [max(1, b // 2) for b in t["sym_bound"]]
-                if num_half:
-                    t["num_bound"] = [max(1, b // 2) for b in t["num_bound"]]
-                assert all(v <= c for v, c in zip(t["sym_bound"], caps["sym_bound"]))
-                assert all(v <= c for v, c in zip(t["num_bound"], caps["num_bound"]))
-                out.append(t)
+                for sym_bitmap in (0, 1):
+                    t = {"sym_bound": list(base["sym_bound"]), "num_bound": list(base["num_bound"]),
+                         "short_hash": short_hash, "sym_bitmap": sym_bitmap}
+                    if sym_half:
+                        t["sym_bound"] = [max(1, b // 2) for b in t["sym_bound"]]
+                    if num_half:
+                        t["num_bound"] = [max(1, b // 2) for b in t["num_bound"]]
+                    assert all(v <= c for v, c in zip(t["sym_bound"], caps["sym_bound"]))
+                    assert all(v <= c for v, c in zip(t["num_bound"], caps["num_bound"]))
+                    out.append(t)
     return out
 
 
@@ -95,9 +97,7 @@ def autotune(A, B, ws: Workspace | None = None, wide: bool = True, reps: int = 2
     best = min(trials, key=lambda r: r["seconds"])
     # Timing noise should not move a workload off the default bounds: the
     # default is kept unless another candidate is more than 2% faster.
-    default = next(r for r in trials if r["tuning"]["sym_bound"] == default_tuning()["sym_bound"]
-                   and r["tuning"]["num_bound"] == default_tuning()["num_bound"]
-                   and r["tuning"]["short_hash"] == default_tuning()["short_hash"])
+    default = next(r for r in trials if r["tuning"] == default_tuning())
     if default["seconds"] <= best["seconds"] * 1.02:
         best = default
     ws.set_tuning(best["tuning"])

@@ -419,6 +419,51 @@ __device__ __forceinline__ void chunk_flat_staged(int bs, int len, V av, Op &op,
     }
 }
 
+// Walks one loaded chunk of cnt A-nonzeros (lane l of the unit holds
+// A-nonzero l's B range [bs, bs + len) and A value av; returns false when
+// the chunk has no product) with the walk its
+// length profile calls for: if the longest B range is at most twice the
+// mean, lanes are grouped per A-nonzero (group width about half the mean,
+// 4..U); otherwise products are flattened across the unit (staged in the
+// unit's LDS `stage` when one is given).  Units that share a chunk always
+// take the staged flat walk when a stage is given.
+template <int U, bool WITH_A, typename V, bool SLICE = false, typename Op>
+__device__ __forceinline__ bool walk_chunk(int cnt, int bs, int len, V av, Op &op, int part, int nparts, bool shared,
+                                           FlatStage<U, V> *stage)
+{
+    const int total = unit_sum<U>(len);
+    const int mx = unit_max<U>(len);
+    if (total == 0) return false;
+    const int mean = (total + cnt - 1) / cnt;
+    if (stage && shared) {
+        // Units sharing a chunk: each takes steps of NP*U consecutive
+        // products across A-nonzero boundaries, instead of one A-nonzero
+        // (two products per lane) per round trip.
+        chunk_flat_staged<U, WITH_A, V>(bs, len, av, op, part, nparts, *stage);
+    } else if (mx <= 2 * mean) {
+        // Groups of about half the mean length: several products per lane per
+        // round amortise the per-round cross-lane reads.
+        const int half = mean >> 1;
+        const int gs = half >= 64 ? 64 : (half >= 32 ? 32 : (half >= 16 ? 16 : (half >= 8 ? 8 : 4)));
+        if (gs <= 4) {
+            chunk_grouped<U, 4, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
+        } else if (gs == 8 || U == 8) {
+            chunk_grouped<U, 8, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
+        } else if (gs == 16 || U == 16) {
+            if constexpr (U >= 16) chunk_grouped<U, 16, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
+        } else if (gs == 32 || U == 32) {
+            if constexpr (U >= 32) chunk_grouped<U, 32, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
+        } else {
+            if constexpr (U >= 64) chunk_grouped<U, 64, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
+        }
+    } else if (stage) {
+        chunk_flat_staged<U, WITH_A, V>(bs, len, av, op, part, nparts, *stage);
+    } else {
+        chunk_flat<U, WITH_A, V>(bs, len, av, op, part, nparts);
+    }
+    return true;
+}
+
 // Calls the consumer for every product of A row segment [a_beg, a_end).
 // `unit` of `nunits` units (all lanes of every unit must call this with the
 // same arguments) share the work in one of two ways:
@@ -430,12 +475,7 @@ __device__ __forceinline__ void chunk_flat_staged(int bs, int len, V av, Op &op,
 // `range(e, k, bs, len)` gives the B entries [bs, bs+len) an A-nonzero e
 // (column k) contributes.
 //
-// Per chunk of U A-nonzeros the walk is chosen at run time: if the longest
-// B row is at most twice the mean, lanes are grouped per A-nonzero (group
-// width about half the mean, 4..U); otherwise products are flattened
-// across the unit (staged in the unit's LDS `stage` when one is given).
-// Units that share a chunk always take the staged flat walk when a stage
-// is given.
+// Each chunk of U A-nonzeros is walked by walk_chunk.
 template <int U, bool WITH_A, typename V, bool SLICE = false, typename Range, typename Op>
 __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit, int nunits,
                                                  const int *__restrict__ acol, const V *__restrict__ aval,
@@ -458,36 +498,7 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
             if constexpr (WITH_A) av = aval[e];
         }
         const int cnt = min(U, a_end - c);
-        const int total = unit_sum<U>(len);
-        const int mx = unit_max<U>(len);
-        if (total == 0) continue;
-        const int mean = (total + cnt - 1) / cnt;
-        if (stage && shared) {
-            // Units sharing a chunk: each takes steps of NP*U consecutive
-            // products across A-nonzero boundaries, instead of one A-nonzero
-            // (two products per lane) per round trip.
-            chunk_flat_staged<U, WITH_A, V>(bs, len, av, op, part, nparts, *stage);
-        } else if (mx <= 2 * mean) {
-            // Groups of about half the mean length: several products per lane per
-            // round amortise the per-round cross-lane reads.
-            const int half = mean >> 1;
-            const int gs = half >= 64 ? 64 : (half >= 32 ? 32 : (half >= 16 ? 16 : (half >= 8 ? 8 : 4)));
-            if (gs <= 4) {
-                chunk_grouped<U, 4, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
-            } else if (gs == 8 || U == 8) {
-                chunk_grouped<U, 8, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
-            } else if (gs == 16 || U == 16) {
-                if constexpr (U >= 16) chunk_grouped<U, 16, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
-            } else if (gs == 32 || U == 32) {
-                if constexpr (U >= 32) chunk_grouped<U, 32, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
-            } else {
-                if constexpr (U >= 64) chunk_grouped<U, 64, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
-            }
-        } else if (stage) {
-            chunk_flat_staged<U, WITH_A, V>(bs, len, av, op, part, nparts, *stage);
-        } else {
-            chunk_flat<U, WITH_A, V>(bs, len, av, op, part, nparts);
-        }
+        if (!walk_chunk<U, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts, shared, stage)) continue;
         if constexpr (has_chunk_end<Op>::value) op.chunk_end(e, e < a_end, shared);
     }
 }

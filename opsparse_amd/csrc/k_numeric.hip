@@ -206,15 +206,20 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
     __shared__ int s_pos;
     __shared__ int s_scan[BS / kWave + 1];
     __shared__ int s_mm[2 * (BS / kWave)];
+    __shared__ BlockStage<BS, V> s_bst;
     constexpr int NU = BS / kWave;
-    constexpr int WPT = T / BS;  // bitmap words per thread in the rank scan
-    static_assert(T % BS == 0, "words per thread");
-    constexpr bool SL = BS > 256;
+    constexpr int WPW = T / NU;  // bitmap words per wave in the rank scan
+    constexpr int RPW = WPW / kWave;
+    static_assert(T % (NU * kWave) == 0, "words per wave");
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
+    const int lane = lane_id();
     const FullRow range{brpt};
 
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
+        // The previous row's output stores complete here, so the walks'
+        // loads are waited for one buffer at a time.
+        drain_vmem();
         const int r = rows ? rows[ridx] : ridx;
         const int a_beg = arpt[r], a_end = arpt[r + 1];
         const long long off = crpt[r];
@@ -246,28 +251,45 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
             const int nw = ((mxc - mn) >> 5) + 1;
             for (int i = tid; i < nw; i += BS) br[i] = make_uint2(0u, 0u);
             __syncthreads();
+            int sn, stot;
+            bool one;
             {
                 RankMark<V> op{bcol, br, mn};
-                for_each_product<kWave, false, V, SL>(a_beg, a_end, wave, NU, acol, aval, range, op);
+                one = block_for_each_product<BS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn, stot);
             }
             __syncthreads();
-            const int wb0 = tid * WPT;
-            int local = 0;
-            for (int i = 0; i < WPT; i++)
-                if (wb0 + i < nw) local += __popc(br[wb0 + i].x);
-            int tot;
-            int run = block_excl_scan<BS>(local, s_scan, tot);
-            for (int i = 0; i < WPT; i++) {
-                if (wb0 + i < nw) {
-                    br[wb0 + i].y = static_cast<unsigned>(run);
-                    run += __popc(br[wb0 + i].x);
-                }
+            // Ranks: wave w owns words [w * WPW, (w + 1) * WPW), its lanes
+            // take 64 consecutive words at a time (conflict-free reads);
+            // DPP scans of the word popcounts, wave totals through LDS.
+            int cnt[RPW], inc[RPW];
+            int wtot = 0;
+#pragma unroll
+            for (int i = 0; i < RPW; i++) {
+                const int w = wave * WPW + i * kWave + lane;
+                cnt[i] = w < nw ? __popc(br[w].x) : 0;
+                inc[i] = wave_incl_scan(cnt[i]);
+                wtot += __builtin_amdgcn_readlane(inc[i], kWave - 1);
+            }
+            if (lane == 0) s_scan[wave] = wtot;
+            __syncthreads();
+            int run = 0;
+            for (int w = 0; w < wave; w++) run += s_scan[w];
+#pragma unroll
+            for (int i = 0; i < RPW; i++) {
+                const int w = wave * WPW + i * kWave + lane;
+                if (w < nw) br[w].y = static_cast<unsigned>(run + inc[i] - cnt[i]);
+                run += __builtin_amdgcn_readlane(inc[i], kWave - 1);
             }
             for (int i = tid; i < n; i += BS) rv[i] = V(0);
             __syncthreads();
             {
                 RankAcc<V> op{bcol, bval, br, rv, mn, 0, static_cast<unsigned>(n)};
-                for_each_product<kWave, true, V, SL>(a_beg, a_end, wave, NU, acol, aval, range, op);
+                // A row that fit in one batch is still staged from the mark.
+                if (one) {
+                    if (stot > 0) block_walk<BS, true, V>(s_bst, sn, stot, op);
+                } else {
+                    block_for_each_product<BS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn, stot);
+                }
             }
             __syncthreads();
             for (int i = tid; i < n; i += BS) cval[off + i] = rv[i];
@@ -295,7 +317,8 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
         if (tid == 0) s_pos = 0;
         __syncthreads();
         NumInsert<T, kHashFib, V> op{bcol, bval, kt, vt};
-        for_each_product<kWave, true, V, SL>(a_beg, a_end, wave, NU, acol, aval, range, op);
+        int sn, stot;
+        block_for_each_product<BS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn, stot);
         __syncthreads();
         for (int i = tid; i < T; i += BS) {
             const int key = kt[i];
@@ -367,10 +390,21 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     if (ridx >= nrows) return;  // whole wave: nothing below synchronises the block
     const int r = rows ? rows[ridx] : ridx;
     const int a_beg = arpt[r], a_end = arpt[r + 1];
+    // A row of at most 64 A-nonzeros is loaded once: lane l keeps its
+    // A-nonzero's B range and value for the span check and both walks.
+    const bool one = a_end - a_beg <= kWave;
+    const int cnt = min(kWave, a_end - a_beg);
+    int cbs = 0, clen = 0;
+    V cav = V(0);
     int mn = INT_MAX, mx = -1;
     for (int e = a_beg + lane; e < a_end; e += kWave) {
         const int k = acol[e];
         const int b0 = brpt[k], b1 = brpt[k + 1];
+        if (one) {
+            cbs = b0;
+            clen = b1 - b0;
+            cav = aval[e];
+        }
         if (b0 < b1) {
             mn = min(mn, bcol[b0]);
             mx = max(mx, bcol[b1 - 1]);
@@ -389,21 +423,22 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     const FullRow range{brpt};
     {
         RankMark<V> op{bcol, L.br, mn};
-        for_each_product<kWave, false, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
+        if (one)
+            walk_chunk<kWave, false, V>(cnt, cbs, clen, cav, op, 0, 1, false, nullptr);
+        else
+            for_each_product<kWave, false, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
     }
     wave_lds_sync();
-    const int wb0 = lane * WPL;
-    int local = 0;
-#pragma unroll
-    for (int i = 0; i < WPL; i++)
-        if (wb0 + i < nw) local += __popc(L.br[wb0 + i].x);
-    int run = unit_incl_scan<kWave>(local) - local;
+    // Ranks: lanes take 64 consecutive words at a time (conflict-free
+    // reads), DPP scans of the word popcounts.
+    int run = 0;
 #pragma unroll
     for (int i = 0; i < WPL; i++) {
-        if (wb0 + i < nw) {
-            L.br[wb0 + i].y = static_cast<unsigned>(run);
-            run += __popc(L.br[wb0 + i].x);
-        }
+        const int w = i * kWave + lane;
+        const int c = w < nw ? __popc(L.br[w].x) : 0;
+        const int inc = wave_incl_scan(c);
+        if (w < nw) L.br[w].y = static_cast<unsigned>(run + inc - c);
+        run += __builtin_amdgcn_readlane(inc, kWave - 1);
     }
     const long long off = crpt[r];
     const int n = min(static_cast<int>(crpt[r + 1] - off), NMAX);
@@ -411,7 +446,10 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     wave_lds_sync();
     {
         RankAcc<V> op{bcol, bval, L.br, L.vals, mn, 0, static_cast<unsigned>(n)};
-        for_each_product<kWave, true, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
+        if (one)
+            walk_chunk<kWave, true, V>(cnt, cbs, clen, cav, op, 0, 1, false, nullptr);
+        else
+            for_each_product<kWave, true, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
     }
     wave_lds_sync();
     for (int i = lane; i < n; i += kWave) cval[off + i] = L.vals[i];

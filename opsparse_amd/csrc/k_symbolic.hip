@@ -38,6 +38,21 @@ struct SymInsert {
     }
 };
 
+// Bitmap of a row with a narrow column span (columns lo .. lo + 32*T - 1):
+// sets the column's bit; the caller counts by popcount.
+struct SymBitmap {
+    const int *__restrict__ bcol;
+    unsigned *bm;
+    int lo;
+    using Item = int;
+    __device__ __forceinline__ int load(int k) const { return bcol[k]; }
+    __device__ __forceinline__ void apply(int col, double, int)
+    {
+        const int c = col - lo;
+        atomicOr(&bm[c >> 5], 1u << (c & 31));
+    }
+};
+
 // Sets the column's bit with an OR that returns nothing (no product waits on
 // an LDS result); the window's count is the bitmap's popcount afterwards.
 struct BitmapInsert {
@@ -55,7 +70,7 @@ struct BitmapInsert {
 
 // G lanes per row (G <= 64: BS/G rows per block; G == BS > 64: one row per
 // block, G/64 waves share the table).
-template <int G, int T, int BS, int H>
+template <int G, int T, int BS, int H, bool BITMAP>
 __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                  const int *__restrict__ brpt, const int *__restrict__ bcol,
                                                  const int *__restrict__ rows, int nrows, int *__restrict__ row_nnz)
@@ -79,11 +94,44 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     int r = 0;
     if (ridx < nrows) {
         r = rows ? rows[ridx] : ridx;
-        SymInsert<T, H> op{bcol, tab + slot * T};
+        const int a_beg = arpt[r], a_end = arpt[r + 1];
         const FullRow range{brpt};
-        for_each_product<U, false, double, true>(arpt[r], arpt[r + 1], unit, NU, acol, static_cast<const double *>(nullptr),
-                                           range, op);
-        cnt = op.cnt;
+        int *stab = tab + slot * T;
+        bool narrow = false;
+        if constexpr (NU == 1 && BITMAP) {
+            // A row whose products span fewer than 32*T columns sets bits
+            // in the slot's T words (an OR per product, no CAS, no
+            // probing) and counts them by popcount.
+            const int gl = tid % G;
+            int mn = INT_MAX, mx = -1;
+            for (int e = a_beg + gl; e < a_end; e += U) {
+                const int k = acol[e];
+                const int bs = brpt[k], be = brpt[k + 1];
+                if (bs < be) {
+                    mn = min(mn, bcol[bs]);
+                    mx = max(mx, bcol[be - 1]);
+                }
+            }
+            mn = unit_min<U>(mn);
+            mx = unit_max<U>(mx);
+            narrow = mx >= 0 && mx - mn < 32 * T;
+            if (narrow) {
+                const int nw = ((mx - mn) >> 5) + 1;
+                for (int i = gl; i < nw; i += U) stab[i] = 0;
+                wave_lds_sync();
+                SymBitmap op{bcol, reinterpret_cast<unsigned *>(stab), mn};
+                for_each_product<U, false, double, true>(a_beg, a_end, unit, NU, acol,
+                                                         static_cast<const double *>(nullptr), range, op);
+                wave_lds_sync();
+                for (int i = gl; i < nw; i += U) cnt += __popc(static_cast<unsigned>(stab[i]));
+            }
+        }
+        if (!narrow) {
+            SymInsert<T, H> op{bcol, stab};
+            for_each_product<U, false, double, true>(a_beg, a_end, unit, NU, acol,
+                                                     static_cast<const double *>(nullptr), range, op);
+            cnt = op.cnt;
+        }
     }
     if constexpr (NU == 1) {
         cnt = unit_sum<U>(cnt);
@@ -177,17 +225,28 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
 // Short-row bins (G <= 64) take the hash family chosen by the tuning; wide
 // rows (bins 5, 6) always hash with the Fibonacci family.
 template <int G, int T, int BS>
-void launch_hash(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash, hipStream_t s)
+void launch_hash(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash, bool bitmap,
+                 hipStream_t s)
 {
     constexpr int RPB = BS / G;
     const int grid = xcd_grid((nrows + RPB - 1) / RPB);
     if constexpr (G <= kWave) {
+        if (bitmap) {
+            if (hash == kHashLin)
+                k_sym_hash<G, T, BS, kHashLin, true>
+                    <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+            else
+                k_sym_hash<G, T, BS, kHashFib, true>
+                    <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+            return;
+        }
         if (hash == kHashLin) {
-            k_sym_hash<G, T, BS, kHashLin><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+            k_sym_hash<G, T, BS, kHashLin, false>
+                <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
             return;
         }
     }
-    k_sym_hash<G, T, BS, kHashFib><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+    k_sym_hash<G, T, BS, kHashFib, false><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
 }
 
 }  // namespace
@@ -201,17 +260,17 @@ void symbolic_init_attributes()
 }
 
 void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash,
-                         hipStream_t s)
+                         bool bitmap, hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
-        case 0: launch_hash<16, 32, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
-        case 1: launch_hash<32, 64, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
-        case 2: launch_hash<64, 128, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
-        case 3: launch_hash<64, 512, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
-        case 4: launch_hash<64, 1024, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
-        case 5: launch_hash<256, 4096, 256>(A, B, rows, nrows, row_nnz, hash, s); break;
-        case 6: launch_hash<1024, 16384, 1024>(A, B, rows, nrows, row_nnz, hash, s); break;
+        case 0: launch_hash<16, 32, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
+        case 1: launch_hash<32, 64, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
+        case 2: launch_hash<64, 128, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
+        case 3: launch_hash<64, 512, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
+        case 4: launch_hash<64, 1024, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
+        case 5: launch_hash<256, 4096, 256>(A, B, rows, nrows, row_nnz, hash, false, s); break;
+        case 6: launch_hash<1024, 16384, 1024>(A, B, rows, nrows, row_nnz, hash, false, s); break;
         default: {
             // Bitmap sized to the column count (up to 2^20 columns per window),
             // so narrow products keep more blocks resident per CU.

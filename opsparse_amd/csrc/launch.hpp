@@ -100,8 +100,9 @@ void launch_narrow_rpt(const long long *crpt64, int M, int *crpt32, hipStream_t 
 // --- symbolic (k_symbolic.hip) ---
 // Computes row_nnz for the rows of one bin.  rows == nullptr means rows 0..nrows-1.
 // hash: family for the short-row bins 0-4 (kHashLin or kHashFib in device.hpp).
+// bitmap: bins 0-4 count rows with a narrow column span in a bitmap.
 void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash,
-                         hipStream_t s);
+                         bool bitmap, hipStream_t s);
 // Bytes of dynamic LDS the long-row symbolic kernel asks for.
 size_t symbolic_long_lds();
 

@@ -91,6 +91,7 @@ struct ops_workspace {
     DevBuf<long long> block_sums, crpt64;
     BinBounds bounds = default_bounds();
     int short_hash = kHashLin;
+    bool sym_bitmap = false;
     std::string err;
 };
 
@@ -245,7 +246,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     t0 = now_s();
     dispatch_bins(ws, s, st.sym_size, ev, [&](int b, hipStream_t ps) {
         launch_symbolic_bin(b, A, B, sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b], ws->row_nnz.p,
-                            pc.hash, ps);
+                            pc.hash, ws->sym_bitmap, ps);
     });
     if (sep) {
         CK(hipStreamSynchronize(s));
@@ -582,6 +583,7 @@ int ops_tuning_default(ops_tuning *t)
         t->num_bound[i] = kNumBound[i];
     }
     t->short_hash = kHashLin;
+    t->sym_bitmap = 0;
     return OPS_OK;
 }
 
@@ -593,6 +595,7 @@ int ops_tuning_caps(ops_tuning *c)
         c->num_bound[i] = kNumCap[i];
     }
     c->short_hash = kHashFib;
+    c->sym_bitmap = 1;
     return OPS_OK;
 }
 
@@ -600,6 +603,7 @@ int ops_workspace_set_tuning(ops_workspace *ws, const ops_tuning *t)
 {
     if (!ws || !t) return OPS_INVALID;
     if (t->short_hash != kHashLin && t->short_hash != kHashFib) return OPS_INVALID;
+    if (t->sym_bitmap != 0 && t->sym_bitmap != 1) return OPS_INVALID;
     for (int i = 0; i < kNumBins - 1; i++) {
         if (t->sym_bound[i] < 0 || t->sym_bound[i] > kSymCap[i]) return OPS_INVALID;
         if (t->num_bound[i] < 0 || t->num_bound[i] > kNumCap[i]) return OPS_INVALID;
@@ -611,6 +615,7 @@ int ops_workspace_set_tuning(ops_workspace *ws, const ops_tuning *t)
         ws->bounds.num[i] = t->num_bound[i];
     }
     ws->short_hash = t->short_hash;
+    ws->sym_bitmap = t->sym_bitmap != 0;
     return OPS_OK;
 }
 
@@ -622,6 +627,7 @@ int ops_workspace_get_tuning(const ops_workspace *ws, ops_tuning *t)
         t->num_bound[i] = ws->bounds.num[i];
     }
     t->short_hash = ws->short_hash;
+    t->sym_bitmap = ws->sym_bitmap ? 1 : 0;
     return OPS_OK;
 }
 

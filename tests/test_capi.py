@@ -101,6 +101,7 @@ def test_tuning_defaults_within_caps_without_gpu():
         assert 0 < d.sym_bound[i] <= c.sym_bound[i] and 0 < d.num_bound[i] <= c.num_bound[i]
         if i:
             assert d.sym_bound[i] >= d.sym_bound[i - 1] and d.num_bound[i] >= d.num_bound[i - 1]
+    assert d.sym_bitmap == 0 and c.sym_bitmap == 1
     assert _lib.lib().ops_workspace_set_tuning(None, C.byref(d)) == _lib.OPS_INVALID
 
 

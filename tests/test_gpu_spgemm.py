@@ -86,8 +86,9 @@ def test_stencil_small(gpu):
 
 def test_short_bins_narrow_and_wide_rows(gpu):
     """Rows with few products, alternating between a narrow column span
-    (banded: the symbolic kernels count them in a bitmap) and a wide one
-    (hashed), in the same launches of bins 0-4."""
+    (banded) and a wide one, in the same launches of bins 0-4; with the
+    symbolic bitmap tuning on, the narrow rows are counted in a bitmap and
+    the wide ones hashed."""
     rng = np.random.default_rng(15)
     N = 20000
     b_rows = []
@@ -101,7 +102,13 @@ def test_short_bins_narrow_and_wide_rows(gpu):
         pool = np.arange(max(0, i - 50), min(N, i + 51)) if i % 2 == 0 else np.arange(N)
         a_rows.append(np.sort(rng.choice(pool, size=min(n, pool.size), replace=False)))
     A = csr_from_rows(N, N, a_rows, rng)
-    run(A, B)
+    from opsparse_amd.spgemm import Workspace, default_tuning
+    dA, dB = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+    for bitmap in (0, 1):
+        ws = Workspace()
+        ws.set_tuning(dict(default_tuning(), sym_bitmap=bitmap))
+        rpt, col, val = spgemm(dA, dB, ws=ws).to_host()
+        check_product(rpt, col, val, A, B)
     run(A, B, np.float32)
 
 
