@@ -161,7 +161,9 @@ struct BwdRankWin {
 };
 
 // Consumer for one sub-window: dC of the product's column at rank - r0.
-template <int U, typename V>
+// The dA term goes to the LDS slot of the batch's A-nonzero (its index e
+// minus the batch start).
+template <typename V>
 struct BwdRank {
     const int *__restrict__ bcol;
     const V *__restrict__ bval;
@@ -169,8 +171,9 @@ struct BwdRank {
     const V *gv;
     int w0;
     int r0;
+    int base;
     V *__restrict__ db;
-    DaReduce<U, V> red;
+    V *sda;
     struct Item {
         int k;
         int c;
@@ -183,11 +186,18 @@ struct BwdRank {
         const int c = it.c - w0;
         const V g = gv[static_cast<int>(e.y) + __popc(e.x & ((1u << (c & 31)) - 1u)) - r0];
         global_add(db + it.k, a * g);
-        red.add(q, g * it.b);
+        atomicAdd(sda + (q - base), g * it.b);
     }
-    __device__ __forceinline__ void chunk_end(int e, bool valid, bool shared) { red.flush(e, valid, shared); }
 };
 
+// Rows above 4096 C entries.  Per window of the C row (WB columns from the
+// window of its next column): the row's C columns set an LDS bitmap, a scan
+// of the word popcounts ranks them (DPP inside each wave, wave totals
+// through LDS), and dC is staged by rank in sub-windows of VCAP entries.
+// Each sub-window walks its products with the whole block (block_stage +
+// block_walk over batches of kLongBS A-nonzeros; B ranges end at the
+// sub-window's last column, found by galloping from per-A-nonzero cursors);
+// after a batch, each thread adds its A-nonzero's dA slot to da.
 template <typename V>
 __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const V *__restrict__ aval, const int *__restrict__ brpt,
@@ -201,17 +211,19 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
     using R = BwdRankWin<V>;
     constexpr int NU = kLongBS / kWave;
     constexpr int WPT = R::WORDS / kLongBS;
+    constexpr int WPW = R::WORDS / NU;
+    constexpr int RPW = WPW / kWave;
     extern __shared__ unsigned long long smem_u64[];
     uint2 *br = reinterpret_cast<uint2 *>(smem_u64);
     V *gv = reinterpret_cast<V *>(br + R::WORDS);
     __shared__ V sda[kLongBS];
     __shared__ int s_cnt[kLongBS / kWave];
-    __shared__ int s_scan[kLongBS / kWave + 1];
+    __shared__ BlockStage<kLongBS, V> s_bst;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
-    const int wb0 = tid * WPT;
+    const int lane = lane_id();
     sda[tid] = V(0);
-    for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
+    for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
     __syncthreads();
 
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
@@ -227,45 +239,64 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
             const int w0 = (ccol[cpos] / R::WB) * R::WB;
             const int wend = (ncols - w0 <= R::WB) ? ncols : w0 + R::WB;
             const long long lim = min(c1, cpos + static_cast<long long>(R::WB));
-            int cnt = 0;
             for (long long j = cpos + tid; j < lim; j += kLongBS) {
                 const int c = ccol[j];
-                if (c < wend) {
-                    atomicOr(&br[(c - w0) >> 5].x, 1u << ((c - w0) & 31));
-                    cnt++;
-                }
-            }
-            cnt = unit_sum<kWave>(cnt);
-            if (lane_id() == 0) s_cnt[wave] = cnt;
-            __syncthreads();
-            int tot = 0;
-            for (int w = 0; w < NU; w++) tot += s_cnt[w];
-            int local = 0;
-            for (int i = 0; i < WPT; i++) local += __popc(br[wb0 + i].x);
-            int t_all;
-            int run = block_excl_scan<kLongBS>(local, s_scan, t_all);
-            for (int i = 0; i < WPT; i++) {
-                br[wb0 + i].y = static_cast<unsigned>(run);
-                run += __popc(br[wb0 + i].x);
+                if (c < wend) atomicOr(&br[(c - w0) >> 5].x, 1u << ((c - w0) & 31));
             }
             __syncthreads();
-            // Sub-windows of up to VCAP entries [s0, s1) of the C row.
+            int ci[RPW], inc[RPW];
+            int wtot = 0;
+#pragma unroll
+            for (int i = 0; i < RPW; i++) {
+                ci[i] = __popc(br[wave * WPW + i * kWave + lane].x);
+                inc[i] = wave_incl_scan(ci[i]);
+                wtot += __builtin_amdgcn_readlane(inc[i], kWave - 1);
+            }
+            if (lane == 0) s_cnt[wave] = wtot;
+            __syncthreads();
+            int tot = 0, run = 0;
+            for (int w = 0; w < NU; w++) {
+                const int t = s_cnt[w];
+                run += w < wave ? t : 0;
+                tot += t;
+            }
+#pragma unroll
+            for (int i = 0; i < RPW; i++) {
+                br[wave * WPW + i * kWave + lane].y = static_cast<unsigned>(run + inc[i] - ci[i]);
+                run += __builtin_amdgcn_readlane(inc[i], kWave - 1);
+            }
+            __syncthreads();
+            // Sub-windows of up to VCAP entries [r0, r1) of the window.
             for (int r0 = 0; r0 < tot; r0 += R::VCAP) {
                 const int r1 = min(tot, r0 + R::VCAP);
                 const int cb = r1 < tot ? ccol[cpos + r1] : wend;
                 for (int i = tid; i < r1 - r0; i += kLongBS) gv[i] = dc[cpos + r0 + i];
                 __syncthreads();
+                drain_vmem();
                 int nxt = INT_MAX;
                 const CursorRange range{brpt, bcol, cursor + (parity ? span : 0), cursor + (parity ? 0 : span), cb,
                                         first, cb >= ncols, &nxt};
-                BwdRank<kWave, V> op{bcol, bval, br, gv, w0, r0, db, DaReduce<kWave, V>{sda + wave * kWave, da}};
-                for_each_product<kWave, true, V, true>(a_beg, a_end, wave, NU, acol, aval, range, op);
-                // Every read of gv (and of s_cnt) is done before the next stage.
+                BwdRank<V> op{bcol, bval, br, gv, w0, r0, 0, db, sda};
+                for (int c = a_beg; c < a_end; c += kLongBS) {
+                    int total;
+                    const int n = block_stage<kLongBS, V>(c, a_end, acol, aval, range, s_bst, total);
+                    op.base = c;
+                    if (total > 0) block_walk<kLongBS, true, V>(s_bst, n, total, op);
+                    const int e = c + tid;
+                    if (e < a_end) {
+                        const V v = sda[tid];
+                        if (v != V(0)) {
+                            da[e] += v;
+                            sda[tid] = V(0);
+                        }
+                    }
+                }
+                // Every read of gv is done before the next stage.
                 __syncthreads();
                 first = false;
                 parity ^= 1;
             }
-            for (int i = 0; i < WPT; i++) br[wb0 + i] = make_uint2(0u, 0u);
+            for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
             __syncthreads();
             cpos += tot;
         }
