@@ -36,16 +36,17 @@ __global__ void k_zero_stats(DevStats *st)
     for (unsigned i = threadIdx.x; i < sizeof(DevStats) / 4; i += blockDim.x) p[i] = 0;
 }
 
-// Products per row, grid-stride over tiles of 256 rows.  Each wave takes
-// 64 rows of the tile in rounds of 8 rows, 8 lanes per row striding over the
-// row's A-nonzeros (coalesced reads of A.col), then an 8-lane sum.  Rows with
-// more than 256 A-nonzeros are summed by the whole block afterwards, so one
-// 40k-entry row neither serialises a group nor stalls its wave.  Per-bin
-// counts accumulate in LDS and reach global memory once per block.
+// Products per row, grid-stride over tiles of 256 rows.  The block walks
+// the tile's A-nonzeros flat (one contiguous range of A.col): in each round
+// thread t takes 4 consecutive entries, so every lane has four independent
+// A.col -> B.rpt load chains in flight whatever the row lengths (a row of
+// 40k A-nonzeros is spread over the whole block), and adds the entries'
+// B-row lengths to its row's LDS sum; the row of an entry comes from a
+// binary search over the tile's row offsets in LDS.  Per-bin counts
+// accumulate in LDS and reach global memory once per block.
 constexpr int kSetupRows = 256;
 constexpr int kSetupMaxGrid = 2048;
-constexpr int kSetupGroup = 8;
-constexpr int kLongRowA = 256;
+constexpr int kSetupUnroll = 4;
 
 __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const int *__restrict__ brpt, int M, int *__restrict__ row_flop,
@@ -57,23 +58,17 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
     __shared__ unsigned long long s_bannz[kNumBins];
     __shared__ unsigned long long s_red[kSetupRows / kWave];
     __shared__ unsigned long long s_f[kSetupRows];
-    __shared__ int s_long[kSetupRows];
-    __shared__ int s_nlong;
+    __shared__ int s_rp[kSetupRows + 1];
     __shared__ int s_max;
 
     const int tid = threadIdx.x;
-    const int lane = lane_id();
-    const int wave = tid / kWave;
-    const int sub = lane & (kSetupGroup - 1);
+    const bool vec = (reinterpret_cast<uintptr_t>(acol) & 15u) == 0;
     if (tid < kNumBins) {
         s_hist[tid] = 0;
         s_bflop[tid] = 0;
         s_bannz[tid] = 0;
     }
-    if (tid == 0) {
-        s_max = 0;
-        s_nlong = 0;
-    }
+    if (tid == 0) s_max = 0;
     __syncthreads();
 
     unsigned long long tot = 0;
@@ -81,41 +76,56 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
     const int ntiles = (M + kSetupRows - 1) / kSetupRows;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int r0 = tile * kSetupRows;
-        for (int round = 0; round < kWave / kSetupGroup; round++) {
-            const int rl = wave * kWave + round * kSetupGroup + lane / kSetupGroup;
-            const int r = r0 + rl;
-            int a0 = 0, a1 = 0;
-            if (r < M) {
-                a0 = arpt[r];
-                a1 = arpt[r + 1];
-            }
-            const bool longrow = a1 - a0 > kLongRowA;
-            unsigned long long part = 0;
-            if (!longrow) {
-                for (int e = a0 + sub; e < a1; e += kSetupGroup) {
-                    const int k = acol[e];
-                    part += static_cast<unsigned long long>(brpt[k + 1] - brpt[k]);
-                }
-            }
-            part = unit_sum<kSetupGroup>(part);
-            if (sub == 0) {
-                s_f[rl] = part;
-                if (longrow) s_long[atomicAdd(&s_nlong, 1)] = rl;
-            }
-        }
+        const int nr = min(kSetupRows, M - r0);
+        s_rp[tid] = arpt[r0 + min(tid, nr)];
+        if (tid == 0) s_rp[kSetupRows] = arpt[r0 + nr];
+        s_f[tid] = 0;
         __syncthreads();
-        const int nlong = s_nlong;
-        for (int i = 0; i < nlong; i++) {
-            const int rl = s_long[i];
-            const int b0 = arpt[r0 + rl], b1 = arpt[r0 + rl + 1];
-            unsigned long long part = 0;
-            for (int e = b0 + tid; e < b1; e += kSetupRows) {
-                const int k = acol[e];
-                part += static_cast<unsigned long long>(brpt[k + 1] - brpt[k]);
+        const int e0 = s_rp[0], e1 = s_rp[nr];
+        // Lane t takes 4 consecutive entries per round (one 16-byte load of
+        // A.col when aligned) and keeps a running sum for its current row,
+        // adding it to the row's LDS sum only when its row changes: a long
+        // row costs one LDS atomic per lane, not one per entry.
+        int cur_row = -1;
+        unsigned long long cur = 0;
+        for (int base = e0 & ~(kSetupUnroll - 1); base < e1; base += kSetupUnroll * kSetupRows) {
+            const int e = base + kSetupUnroll * tid;
+            int k[kSetupUnroll];
+            if (vec && e >= e0 && e + kSetupUnroll <= e1) {
+                const int4 v = *reinterpret_cast<const int4 *>(acol + e);
+                k[0] = v.x;
+                k[1] = v.y;
+                k[2] = v.z;
+                k[3] = v.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < kSetupUnroll; j++) k[j] = e + j >= e0 && e + j < e1 ? acol[e + j] : -1;
             }
-            part = block_sum<kSetupRows>(part, s_red);
-            if (tid == 0) s_f[rl] = part;
+            int f[kSetupUnroll];
+#pragma unroll
+            for (int j = 0; j < kSetupUnroll; j++) f[j] = k[j] >= 0 ? brpt[k[j] + 1] - brpt[k[j]] : 0;
+#pragma unroll
+            for (int j = 0; j < kSetupUnroll; j++) {
+                if (k[j] < 0) continue;
+                const int ej = e + j;
+                if (cur_row < 0 || ej >= s_rp[cur_row + 1]) {
+                    if (cur) atomicAdd(&s_f[cur_row], cur);
+                    cur = 0;
+                    // Largest row rl < nr with s_rp[rl] <= ej.
+                    int lo = cur_row < 0 ? 0 : cur_row, hi = nr - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_rp[mid] <= ej)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    cur_row = lo;
+                }
+                cur += static_cast<unsigned long long>(f[j]);
+            }
         }
+        if (cur) atomicAdd(&s_f[cur_row], cur);
         __syncthreads();
         const int r = r0 + tid;
         if (r < M) {
@@ -128,12 +138,11 @@ __global__ __launch_bounds__(kSetupRows) void k_setup(const int *__restrict__ ar
                 const int b = dev_sym_bin(fi, bb);
                 atomicAdd(&s_hist[b], 1);
                 atomicAdd(&s_bflop[b], f);
-                atomicAdd(&s_bannz[b], static_cast<unsigned long long>(arpt[r + 1] - arpt[r]));
+                atomicAdd(&s_bannz[b], static_cast<unsigned long long>(s_rp[tid + 1] - s_rp[tid]));
             }
             mx = max(mx, fi);
             tot += f;
         }
-        if (tid == 0) s_nlong = 0;
         __syncthreads();
     }
     atomicMax(&s_max, mx);
