@@ -314,133 +314,20 @@ __device__ __forceinline__ void chunk_flat(int bs, int len, V av, Op &op, int pa
     }
 }
 
-// Per-unit LDS staging of a chunk for the staged flat walk.
-template <int U, typename V>
-struct FlatStage {
-    int excl[U + 1];
-    int bs[U];
-    V av[U];
-};
-
-// Flattened walk with the chunk staged in LDS: the scanned B-range offsets,
-// B starts and A values of the chunk's A-nonzeros go to the unit's LDS
-// stage, and each lane walks its products (l, l+U, ... of this part's
-// steps, NP per step) with a running A-nonzero index that only moves
-// forward: one LDS read per product while it stays in the same B range, a
-// binary search over the staged offsets when it leaves it.  Compared with
-// chunk_flat this replaces a log2(U)-deep chain of cross-lane reads per
-// product by one LDS read, and keeps the NP loads of the next step in flight
-// during this step's updates; it pays off on chunks of long, skewed B
-// ranges (power-law rows).
-template <int U, bool WITH_A, typename V, typename Op>
-__device__ __forceinline__ void chunk_flat_staged(int bs, int len, V av, Op &op, int part, int nparts,
-                                                  FlatStage<U, V> &st)
-{
-    constexpr int NP = 4;
-    const int lane = lane_id();
-    const int gl = lane & (U - 1);
-    const int gbase = lane & ~(U - 1);
-    const int incl = unit_incl_scan<U>(len);
-    const int total = bperm(incl, gbase + U - 1);
-    wave_lds_sync();  // the previous chunk's stage is no longer read
-    st.excl[gl] = incl - len;
-    st.bs[gl] = bs;
-    if constexpr (WITH_A) st.av[gl] = av;
-    if (gl == U - 1) st.excl[U] = total;
-    wave_lds_sync();
-    int q = 0;
-    int q_lo = 0, q_hi = st.excl[1], q_bs = st.bs[0];
-    V q_av = WITH_A ? st.av[0] : V(0);
-    // Positions of step `base`: B index, A value and A-nonzero of each product.
-    auto gather = [&](int base, int (&bb)[NP], V (&aa)[NP], int (&qq)[NP], bool (&ok)[NP]) {
-#pragma unroll
-        for (int n = 0; n < NP; n++) {
-            const int p = base + n * U + gl;
-            ok[n] = p < total;
-            if (ok[n]) {
-                if (p >= q_hi) {
-                    // Largest q' > q with excl[q'] <= p (excl[U] = total > p).
-                    int lo = q + 1, hi = U - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (st.excl[mid] <= p)
-                            lo = mid;
-                        else
-                            hi = mid - 1;
-                    }
-                    q = lo;
-                    q_lo = st.excl[q];
-                    q_hi = st.excl[q + 1];
-                    q_bs = st.bs[q];
-                    if constexpr (WITH_A) q_av = st.av[q];
-                }
-                bb[n] = q_bs + (p - q_lo);
-                aa[n] = q_av;
-                qq[n] = q;
-            }
-        }
-    };
-    // Software pipeline: the B loads of the next step are in flight while
-    // this step's updates run.
-    const int stride = nparts * NP * U;
-    int base = part * NP * U;
-    if (base >= total) return;
-    int bb[NP], qq[NP];
-    V aa[NP];
-    bool ok[NP];
-    typename Op::Item it[NP];
-    gather(base, bb, aa, qq, ok);
-#pragma unroll
-    for (int n = 0; n < NP; n++)
-        if (ok[n]) it[n] = op.load(bb[n]);
-    while (true) {
-        const int nb = base + stride;
-        const bool more = nb < total;
-        int bb2[NP], qq2[NP];
-        V aa2[NP];
-        bool ok2[NP];
-        typename Op::Item it2[NP];
-        if (more) {
-            gather(nb, bb2, aa2, qq2, ok2);
-#pragma unroll
-            for (int n = 0; n < NP; n++)
-                if (ok2[n]) it2[n] = op.load(bb2[n]);
-        }
-        apply_batch<NP>(op, it, aa, qq, ok);
-        if (!more) break;
-        base = nb;
-#pragma unroll
-        for (int n = 0; n < NP; n++) {
-            it[n] = it2[n];
-            aa[n] = aa2[n];
-            qq[n] = qq2[n];
-            ok[n] = ok2[n];
-        }
-    }
-}
-
 // Walks one loaded chunk of cnt A-nonzeros (lane l of the unit holds
 // A-nonzero l's B range [bs, bs + len) and A value av; returns false when
 // the chunk has no product) with the walk its
 // length profile calls for: if the longest B range is at most twice the
 // mean, lanes are grouped per A-nonzero (group width about half the mean,
-// 4..U); otherwise products are flattened across the unit (staged in the
-// unit's LDS `stage` when one is given).  Units that share a chunk always
-// take the staged flat walk when a stage is given.
+// 4..U); otherwise products are flattened across the unit.
 template <int U, bool WITH_A, typename V, bool SLICE = false, typename Op>
-__device__ __forceinline__ bool walk_chunk(int cnt, int bs, int len, V av, Op &op, int part, int nparts, bool shared,
-                                           FlatStage<U, V> *stage)
+__device__ __forceinline__ bool walk_chunk(int cnt, int bs, int len, V av, Op &op, int part, int nparts)
 {
     const int total = unit_sum<U>(len);
     const int mx = unit_max<U>(len);
     if (total == 0) return false;
     const int mean = (total + cnt - 1) / cnt;
-    if (stage && shared) {
-        // Units sharing a chunk: each takes steps of NP*U consecutive
-        // products across A-nonzero boundaries, instead of one A-nonzero
-        // (two products per lane) per round trip.
-        chunk_flat_staged<U, WITH_A, V>(bs, len, av, op, part, nparts, *stage);
-    } else if (mx <= 2 * mean) {
+    if (mx <= 2 * mean) {
         // Groups of about half the mean length: several products per lane per
         // round amortise the per-round cross-lane reads.
         const int half = mean >> 1;
@@ -456,8 +343,6 @@ __device__ __forceinline__ bool walk_chunk(int cnt, int bs, int len, V av, Op &o
         } else {
             if constexpr (U >= 64) chunk_grouped<U, 64, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
         }
-    } else if (stage) {
-        chunk_flat_staged<U, WITH_A, V>(bs, len, av, op, part, nparts, *stage);
     } else {
         chunk_flat<U, WITH_A, V>(bs, len, av, op, part, nparts);
     }
@@ -479,7 +364,7 @@ __device__ __forceinline__ bool walk_chunk(int cnt, int bs, int len, V av, Op &o
 template <int U, bool WITH_A, typename V, bool SLICE = false, typename Range, typename Op>
 __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit, int nunits,
                                                  const int *__restrict__ acol, const V *__restrict__ aval,
-                                                 const Range &range, Op &op, FlatStage<U, V> *stage = nullptr)
+                                                 const Range &range, Op &op)
 {
     const int lane = lane_id();
     const int gl = lane & (U - 1);
@@ -498,7 +383,7 @@ __device__ __forceinline__ void for_each_product(int a_beg, int a_end, int unit,
             if constexpr (WITH_A) av = aval[e];
         }
         const int cnt = min(U, a_end - c);
-        if (!walk_chunk<U, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts, shared, stage)) continue;
+        if (!walk_chunk<U, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts)) continue;
         if constexpr (has_chunk_end<Op>::value) op.chunk_end(e, e < a_end, shared);
     }
 }
@@ -688,14 +573,15 @@ __device__ __forceinline__ void block_walk(const BlockStage<BS, V> &st, int n, i
         };
         load_q();
         // Steps past the slice, and lanes past its end, get ok = false and
-        // B index 0 (a valid entry: the batch has products), so every load
-        // below is issued unconditionally.
+        // the B index of the batch's first product (a valid entry), so
+        // every load below is issued unconditionally.
+        const int safe = st.ent[0].off;
         auto gather = [&](int b0, int (&bi)[NP], V (&aa)[NP], int (&qq)[NP], bool (&ok)[NP]) {
 #pragma unroll
             for (int s = 0; s < NP; s++) {
                 const int sb = b0 + s * kWave;
                 ok[s] = false;
-                bi[s] = 0;
+                bi[s] = safe;
                 aa[s] = V(0);
                 qq[s] = 0;
                 if (sb < s_end) {

@@ -424,7 +424,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     {
         RankMark<V> op{bcol, L.br, mn};
         if (one)
-            walk_chunk<kWave, false, V>(cnt, cbs, clen, cav, op, 0, 1, false, nullptr);
+            walk_chunk<kWave, false, V>(cnt, cbs, clen, cav, op, 0, 1);
         else
             for_each_product<kWave, false, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
     }
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     {
         RankAcc<V> op{bcol, bval, L.br, L.vals, mn, 0, static_cast<unsigned>(n)};
         if (one)
-            walk_chunk<kWave, true, V>(cnt, cbs, clen, cav, op, 0, 1, false, nullptr);
+            walk_chunk<kWave, true, V>(cnt, cbs, clen, cav, op, 0, 1);
         else
             for_each_product<kWave, true, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
     }
