@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-check", action="store_true", help="skip output checks")
     p.add_argument("--batch-count", type=int, default=4096)
     p.add_argument("--fd-entries", type=int, default=1024)
+    p.add_argument("--vendor", action="store_true", help="also time rocSPARSE SpGEMM (default for stencil/uniform)")
+    p.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparison")
     p.add_argument("--copy-calibration", action="store_true",
                    help="after the run, copy exactly 1 GiB device-to-device (the known-size kernel that "
                         "FETCH_SIZE / WRITE_SIZE counter runs are scaled by)")
@@ -194,6 +196,33 @@ def line(value, steps, warmup, ms_per_step, dtype, cfg, roof, cpu):
 
 # ----------------------------------------------------------------------------- plain products
 
+def vendor_leg(dA, dB, dtype_f32, reps, ours_ms, ours_nnz):
+    """rocSPARSE SpGEMM on the same device inputs (benchmark-only comparator
+    in its own library, libvendor_rocsparse.so): mean ms per complete product
+    over `reps` calls after one warm-up, with C allocated inside the call as
+    for ours.  Also checks that its nnz(C) equals ours."""
+    import ctypes as C
+    import os
+    from opsparse_amd._lib import OPS_F32, OPS_F64, OpsCsr
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "opsparse_amd", "libvendor_rocsparse.so")
+    if not os.path.exists(path):
+        return {"available": False, "reason": "libvendor_rocsparse.so not built"}
+    lib = C.CDLL(path)
+    f = lib.vendor_rocsparse_spgemm
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(OpsCsr), C.POINTER(OpsCsr), C.c_int, C.c_int, C.POINTER(C.c_double),
+                  C.POINTER(C.c_int64), C.POINTER(OpsCsr)]
+    a, b = dA.desc(), dB.desc()
+    sec, nnz = C.c_double(), C.c_int64()
+    st = f(C.byref(a), C.byref(b), OPS_F32 if dtype_f32 else OPS_F64, reps, C.byref(sec), C.byref(nnz), None)
+    if st != 0:
+        return {"available": False, "reason": f"rocsparse_spgemm failed (status {st})"}
+    ms = sec.value * 1e3
+    return {"available": True, "library": "rocSPARSE generic SpGEMM (rocsparse_spgemm, alg default)",
+            "ms_per_call": round(ms, 4), "nnz_c": int(nnz.value), "nnz_matches": int(nnz.value) == int(ours_nnz),
+            "speedup_ours_vs_rocsparse": round(ms / ours_ms, 2), "calls": reps}
+
+
 def run_single(args, name):
     """stencil / uniform / rmat20: one drop-in call per step."""
     import numpy as np
@@ -231,6 +260,10 @@ def run_single(args, name):
            "gen_s": round(t_gen, 2), "ref_convention_ms": round(float(np.mean([t.total for t in tims])) * 1e3, 4),
            "pipeline_roofline_frac": round(bytes_alg / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
            "bytes_alg": int(bytes_alg), "phases": phase_ms(tims), "parallelism": "single GPU"}
+    if args.vendor or (name != "rmat20" and not args.no_vendor):
+        torch.cuda.synchronize()
+        cfg["vendor_rocsparse"] = vendor_leg(dA, dA, False, max(1, min(args.steps, 5)), ms_per_step, nnz_c)
+        log(f"{name}: rocSPARSE {cfg['vendor_rocsparse']}")
     cpu = None
     if not args.no_cpu_baseline:
         if name == "rmat20":
@@ -522,6 +555,10 @@ def run_backward(args):
            "fwd_plus_bwd_over_fwd": round((t_fwd + t_bwd) / t_fwd, 3),
            "numeric_plus_bwd_over_numeric": round((t_num + t_bwd) / t_num, 3), "phases": phase_ms(tims),
            "parallelism": "single GPU"}
+    if not args.no_vendor:
+        torch.cuda.synchronize()
+        cfg["vendor_rocsparse_forward"] = vendor_leg(dA, dB, True, 3, t_fwd * 1e3, plan.nnz)
+        log(f"backward: rocSPARSE forward {cfg['vendor_rocsparse_forward']}")
     if not args.no_check:
         crpt_h, ccol_h = rpt.cpu().numpy(), col.cpu().numpy()
         dA64, dB64 = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)

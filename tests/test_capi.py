@@ -111,3 +111,18 @@ def test_mem_cache_counters_without_gpu():
     assert st.cached_bytes == 0 and st.live_bytes == 0 and st.limit == -1
     assert _lib.lib().ops_mem_cache_stats(None) == _lib.OPS_INVALID
     assert _lib.lib().ops_free(None) == _lib.OPS_OK  # like hipFree(NULL)
+
+
+def test_vendor_comparator_is_a_separate_library():
+    """The rocSPARSE comparator is its own library with one entry point, and
+    the SpGEMM library does not link rocSPARSE (it is never a dispatch path)."""
+    import os
+    import subprocess
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    vend = os.path.join(here, "opsparse_amd", "libvendor_rocsparse.so")
+    main = os.path.join(here, "opsparse_amd", "libopsparse_amd.so")
+    assert os.path.exists(vend) and os.path.exists(main)
+    syms = subprocess.run(["nm", "-D", "--defined-only", vend], capture_output=True, text=True, check=True).stdout
+    assert "vendor_rocsparse_spgemm" in syms
+    deps = subprocess.run(["readelf", "-d", main], capture_output=True, text=True, check=True).stdout
+    assert "rocsparse" not in deps
