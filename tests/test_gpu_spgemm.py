@@ -185,6 +185,50 @@ def test_plan_and_graph(gpu):
     oracle.compare(rpt, col_g.cpu().numpy(), val_g.cpu().numpy(), oracle.spgemm(A2, A2), TOL[np.float64])
 
 
+def test_plan_graph_long_spill_and_short_rows(gpu):
+    """A plan whose numeric phase runs the long-row kernel over several
+    windows (with B's window-cut table), the wave kernel of bin 5, its spill
+    list into the block kernel, and short bins, eagerly and as a replayed
+    hipGraph (values changed between replays)."""
+    rng = np.random.default_rng(16)
+    W = 1 << 18
+    N = 5 * W + 7
+    Kw, Kn = 1100, 100
+    wide = random_csr(Kw, 3 * W, rng.integers(250, 330, Kw), rng)
+    col = wide.col.astype(np.int64)
+    col = np.array([0, 2, 5], dtype=np.int64)[col // W] * W + col % W
+    rows = [np.minimum(col[wide.rpt[i]:wide.rpt[i + 1]], N - 1) for i in range(Kw)]
+    rows += [np.sort(rng.choice(np.arange(1000, 9000), size=int(rng.integers(60, 100)), replace=False))
+             for _ in range(Kn)]
+    B = csr_from_rows(Kw + Kn, N, rows, rng)
+    a_rows = ([np.sort(rng.choice(Kw, size=int(rng.integers(300, 420)), replace=False)) for _ in range(3)]
+              + [np.sort(rng.choice(np.arange(Kw, Kw + Kn), size=int(rng.integers(6, 10)), replace=False))
+                 for _ in range(3)]
+              + [np.sort(rng.choice(Kw, size=int(rng.integers(2, 4)), replace=False)) for _ in range(3)]
+              + [np.array([int(rng.integers(0, Kw))]) for _ in range(3)])
+    A = csr_from_rows(len(a_rows), Kw + Kn, a_rows, rng)
+    dA, dB = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+    from opsparse_amd._lib import OpsTimings
+    t = OpsTimings()
+    spgemm(dA, dB, timings=t).free()
+    assert t.num_rows[7] == 3 and t.num_rows[5] >= 3 and sum(t.num_rows[:5]) >= 3, list(t.num_rows)
+    p = Plan(dA, dB)
+    rpt = p.rowptr64().cpu().numpy()
+    ref = oracle.spgemm(A, B)
+    col_e, val_e = p.numeric(dA, dB)
+    oracle.compare(rpt, col_e.cpu().numpy(), val_e.cpu().numpy(), ref, TOL[np.float64])
+    col_g = torch.empty(p.nnz, dtype=torch.int32, device="cuda")
+    val_g = torch.empty(p.nnz, dtype=torch.float64, device="cuda")
+    for scale in (1.0, -0.5):
+        A2 = G.HostCSR(A.M, A.N, A.rpt, A.col, A.val * scale)
+        dA.val.copy_(DeviceCSR.from_host(A2, torch.float64).val)
+        for _ in range(2):
+            val_g.fill_(7.0)
+            p.numeric(dA, dB, col_g, val_g, graph=True)
+            torch.cuda.synchronize()
+            oracle.compare(rpt, col_g.cpu().numpy(), val_g.cpu().numpy(), oracle.spgemm(A2, B), TOL[np.float64])
+
+
 def test_output_cache_reuse_and_limit(gpu):
     A = G.rmat(12, 10, 8)
     dA = DeviceCSR.from_host(A, torch.float64)
