@@ -15,10 +15,11 @@ Other workloads (--workload):
   backward : R-MAT 2^18 A*B in FP32, forward + masked backward, gradients
              checked against FP64 direct sums and FP64 finite differences
 
-Prints ONE JSON line.  value = 2 * products / time in GFLOP/s.  roofline =
-the kernel with the largest mean HIP-event time over the timed calls (events
-recorded on the pool stream it runs on), with its algorithmic bytes per
-launch:
+Prints ONE JSON line.  value = 2 * products / time in GFLOP/s, time = the
+mean of the timed calls, each bracketed by synchronisations.  roofline = the
+kernel with the largest mean HIP-event time over three diagnostic calls made
+after the timed ones (events recorded on the pool stream it runs on; the
+timed calls record no events), with its algorithmic bytes per launch:
   numeric bin  : (4+w)*P + (4+w)*Z + (12+w)*nnzA + 28*rows
   symbolic bin : 4*P + 12*nnzA + 16*rows
 (P products, Z output entries, nnzA A-nonzeros and rows of that bin; w the
@@ -40,7 +41,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "SpGEMM GFLOP/s (2×intermediate products/s), % HBM roofline; batched & fwd+bwd"
-KEEP_BYTES = 64 << 30  # keep every timed output alive (freed after timing) below this total
 
 
 def parse():
@@ -76,7 +76,7 @@ def kernel_bytes(kind, b, t, w):
 
 
 def roofline_of(timings, w, workload_tag):
-    """Dominant kernel over the timed calls, from their per-bin event times."""
+    """Dominant kernel over the diagnostic calls, from their per-bin event times."""
     import numpy as np
     agg = {}
     for t in timings:
@@ -99,51 +99,44 @@ def roofline_of(timings, w, workload_tag):
     return roof
 
 
-def timed_calls(A, B, ws, steps, warmup, wide):
-    """warmup + steps drop-in calls; returns (timed seconds, per-call timings, last C).
+def timed_calls(A, B, ws, steps, warmup, wide, diag_calls=3):
+    """warmup + steps drop-in calls; returns (timed seconds, timings of the diagnostic
+    calls, last C, host timings of the timed calls).
 
-    When all outputs fit in KEEP_BYTES the steps run back to back and the
-    outputs are released after the timed region; otherwise each step is
-    bracketed by synchronisations and its output released between steps,
-    outside the timed spans (the seconds are then their sum).  The output
-    size comes from the first warm-up call (one untimed call is made for it
-    when warmup is 0)."""
+    As in the reference's profile loop (benchmarks/opsparse_profile.cu:24-39),
+    each timed call is bracketed by synchronisations and its output is
+    released between calls, outside the timed spans; the seconds are their
+    sum.  The timed calls record host phase times only.  Per-bin kernel
+    event times (a diagnostic whose event records and waits would add to
+    the call) come from `diag_calls` further untimed calls, whose timings
+    are the ones returned (phases and the roofline kernel read them)."""
     import torch
     from opsparse_amd._lib import OpsTimings
     from opsparse_amd.spgemm import spgemm
-    out_bytes = 0
     for _ in range(max(warmup, 1)):
-        c = spgemm(A, B, ws=ws, wide=wide)
-        out_bytes = c.nnz * (4 + A.val.element_size()) + (c.M + 1) * (8 if wide else 4)
-        c.free()
+        spgemm(A, B, ws=ws, wide=wide).free()
     torch.cuda.synchronize()
-    tims = []
-    if out_bytes * steps <= KEEP_BYTES:
-        outs = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            t = OpsTimings()
-            t.kernel_events = 1
-            outs.append(spgemm(A, B, ws=ws, timings=t, wide=wide))
-            tims.append(t)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        for c in outs[:-1]:
-            c.free()
-        return elapsed, tims, outs[-1]
     elapsed, last = 0.0, None
+    timed = []
     for _ in range(steps):
         if last is not None:
             last.free()
         t = OpsTimings()
-        t.kernel_events = 1
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         last = spgemm(A, B, ws=ws, timings=t, wide=wide)
         torch.cuda.synchronize()
         elapsed += time.perf_counter() - t0
+        timed.append(t)
+    tims = []
+    for _ in range(diag_calls):
+        last.free()
+        t = OpsTimings()
+        t.kernel_events = 1
+        last = spgemm(A, B, ws=ws, timings=t, wide=wide)
+        torch.cuda.synchronize()
         tims.append(t)
-    return elapsed, tims, last
+    return elapsed, tims, last, timed
 
 
 def cpu_leg(A, B, r0, r1, iters, desc, gpu_c, tol, check):
@@ -175,11 +168,13 @@ def cpu_leg(A, B, r0, r1, iters, desc, gpu_c, tol, check):
     return cpu, err
 
 
-def phase_ms(tims):
-    """Mean host-side phase times (ms) and the last call's per-bin kernel event times (the symbolic
-    kernels are waited for through their events, so "scan" is the scan alone)."""
+def phase_ms(tims, host=None):
+    """Mean host-side phase times (ms) of `host` (default: tims) and the last of `tims`' per-bin
+    kernel event times (in calls with events the symbolic kernels are waited for through their
+    events, so "scan" is the scan alone)."""
     import numpy as np
-    mean = lambda f: round(float(np.mean([getattr(t, f) for t in tims])) * 1e3, 3)
+    host = host or tims
+    mean = lambda f: round(float(np.mean([getattr(t, f) for t in host])) * 1e3, 3)
     return {"setup": mean("setup"), "scan": mean("prefix"), "allocate_c": mean("allocate"),
             "numeric": mean("numeric"), "total": mean("total"),
             "symbolic_bin_ms": [round(x, 3) for x in tims[-1].sym_ms],
@@ -249,7 +244,7 @@ def run_single(args, name):
     ws = Workspace()
     torch.cuda.synchronize()
     log(f"{name}: rows {A.M} nnz {A.nnz} products {P}")
-    elapsed, tims, c_last = timed_calls(dA, dA, ws, args.steps, args.warmup, wide)
+    elapsed, tims, c_last, timed = timed_calls(dA, dA, ws, args.steps, args.warmup, wide)
     nnz_c = c_last.nnz
     ms_per_step = elapsed / args.steps * 1e3
     gflops = 2.0 * P / (elapsed / args.steps) / 1e9
@@ -257,9 +252,9 @@ def run_single(args, name):
     roof = roofline_of(tims, w, name)
     bytes_alg = (8 + w) * P + (4 + w) * nnz_c + (36 + w) * A.nnz + 32 * A.M
     cfg = {"workload": desc, "rows": A.M, "nnz_a": A.nnz, "products": int(P), "nnz_c": int(nnz_c),
-           "gen_s": round(t_gen, 2), "ref_convention_ms": round(float(np.mean([t.total for t in tims])) * 1e3, 4),
+           "gen_s": round(t_gen, 2), "ref_convention_ms": round(float(np.mean([t.total for t in timed])) * 1e3, 4),
            "pipeline_roofline_frac": round(bytes_alg / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
-           "bytes_alg": int(bytes_alg), "phases": phase_ms(tims), "parallelism": "single GPU"}
+           "bytes_alg": int(bytes_alg), "phases": phase_ms(tims, timed), "parallelism": "single GPU"}
     if args.vendor or (name != "rmat20" and not args.no_vendor):
         torch.cuda.synchronize()
         cfg["vendor_rocsparse"] = vendor_leg(dA, dA, False, max(1, min(args.steps, 5)), ms_per_step, nnz_c)
@@ -313,7 +308,7 @@ def run_batch(args):
         dB = DeviceCSR.from_host(Bh, dt)
         tune = autotune.autotune(dA, dB, ws=ws, wide=True, reps=3)
         log(f"batch {tag}: tuning {tune['tuning']} (cached: {tune['from_cache']})")
-        elapsed, tims, c_last = timed_calls(dA, dB, ws, args.steps, args.warmup, True)
+        elapsed, tims, c_last, _ = timed_calls(dA, dB, ws, args.steps, args.warmup, True)
         t_fused = elapsed / args.steps
         nnz_c = c_last.nnz
         log(f"batch {tag}: fused {t_fused * 1e3:.3f} ms, nnz(C) {nnz_c}")
@@ -521,7 +516,7 @@ def run_backward(args):
     A32, B32 = A.astype(np.float32), B.astype(np.float32)
     dA = DeviceCSR.from_host(A32, torch.float32)
     dB = DeviceCSR.from_host(B32, torch.float32)
-    elapsed, tims, c_last = timed_calls(dA, dB, ws, args.steps, args.warmup, True)
+    elapsed, tims, c_last, _ = timed_calls(dA, dB, ws, args.steps, args.warmup, True)
     t_fwd = elapsed / args.steps
     plan = Plan(dA, dB, ws=ws)
     rpt = plan.rowptr64()
