@@ -251,10 +251,16 @@ def run_single(args, name):
     log(f"{name}: {ms_per_step:.3f} ms/call, {gflops:.1f} GFLOP/s, nnz(C) {nnz_c}")
     roof = roofline_of(tims, w, name)
     bytes_alg = (8 + w) * P + (4 + w) * nnz_c + (36 + w) * A.nnz + 32 * A.M
+    # Compulsory bytes: read A and B once, write C once (a lower bound no
+    # SpGEMM can go below).
+    bytes_comp = (4 + w) * (2 * A.nnz + nnz_c) + 4 * (A.M + 1) * 3
+    t_step = elapsed / args.steps
     cfg = {"workload": desc, "rows": A.M, "nnz_a": A.nnz, "products": int(P), "nnz_c": int(nnz_c),
            "gen_s": round(t_gen, 2), "ref_convention_ms": round(float(np.mean([t.total for t in timed])) * 1e3, 4),
-           "pipeline_roofline_frac": round(bytes_alg / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
-           "bytes_alg": int(bytes_alg), "phases": phase_ms(tims, timed), "parallelism": "single GPU"}
+           "pipeline_roofline_frac": round(bytes_alg / t_step / 1e9 / HBM_PEAK_GBS, 4),
+           "bytes_alg": int(bytes_alg), "compulsory_bytes": int(bytes_comp),
+           "compulsory_roofline_frac": round(bytes_comp / t_step / 1e9 / HBM_PEAK_GBS, 4),
+           "phases": phase_ms(tims, timed), "parallelism": "single GPU"}
     if args.vendor or (name != "rmat20" and not args.no_vendor):
         torch.cuda.synchronize()
         cfg["vendor_rocsparse"] = vendor_leg(dA, dA, False, max(1, min(args.steps, 5)), ms_per_step, nnz_c)
