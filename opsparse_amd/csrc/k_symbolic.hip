@@ -145,8 +145,7 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
 }
 
 constexpr int kLongBS = 1024;
-constexpr int kBitmapWords = 32768;  // 128 KB: 2^20 columns per window
-constexpr int kBitmapCols = kBitmapWords * 32;
+constexpr int kBitmapWords = kSymLongWindow / 32;  // 128 KB
 
 // B range of an A-nonzero restricted to columns [w0, wend).  For a single
 // window covering all columns this is the whole row.

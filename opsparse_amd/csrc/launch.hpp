@@ -98,6 +98,8 @@ void launch_scan(const int *row_nnz, const int *row_flop, const int *arpt, int M
 void launch_narrow_rpt(const long long *crpt64, int M, int *crpt32, hipStream_t s);
 
 // --- symbolic (k_symbolic.hip) ---
+// Columns per bitmap window of the long-row symbolic kernel (128 KB of LDS).
+constexpr int kSymLongWindow = 1 << 20;
 // Computes row_nnz for the rows of one bin.  rows == nullptr means rows 0..nrows-1.
 // hash: family for the short-row bins 0-4 (kHashLin or kHashFib in device.hpp).
 // bitmap: bins 0-4 count rows with a narrow column span in a bitmap.

@@ -195,7 +195,13 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     pc.N = B.N;
     pc.crpt64 = crpt64_dst;
     pc.hash = ws->short_hash;
-    const BinBounds &bb = ws->bounds;
+    // When B's columns fit one bitmap window of the long-row symbolic
+    // kernel, rows above the bin-5 bound go to it (bin 7) instead of the
+    // 16384-slot hash of bin 6: setting bits costs no probing or CAS, and
+    // the one window is walked once.  With more columns every window would
+    // walk the row again, and bin 6 keeps hashing.
+    BinBounds bb = ws->bounds;
+    if (B.N <= kSymLongWindow) bb.sym[kNumBins - 2] = bb.sym[kNumBins - 3];
     double t0 = now_s();
 
     ws->row_flop.ensure(static_cast<size_t>(M) + 1);

@@ -141,6 +141,26 @@ def test_symbolic_multi_window(gpu):
     run(A, B)
 
 
+def test_symbolic_bin6_hash_or_bitmap(gpu):
+    """Rows of 4097..16384 products: counted by the bin-6 hash kernel when B
+    has more columns than one long-row bitmap window (2^20), by the bitmap
+    kernel (bin 7) otherwise."""
+    from opsparse_amd._lib import OpsTimings
+    rng = np.random.default_rng(17)
+    K = 400
+    A = random_csr(16, K, rng.integers(30, 40, 16), rng)
+    for N in ((1 << 21) + 5, 300000):
+        B = random_csr(K, N, rng.integers(150, 300, K), rng)
+        dA, dB = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+        t = OpsTimings()
+        spgemm(dA, dB, timings=t).free()
+        if N > (1 << 20):
+            assert t.sym_rows[6] > 0, list(t.sym_rows)
+        else:
+            assert t.sym_rows[6] == 0 and t.sym_rows[7] > 0, list(t.sym_rows)
+        run(A, B)
+
+
 def test_wide_rowptr(gpu):
     A = G.rmat(11, 8, 5)
     run(A, A, wide=True)
