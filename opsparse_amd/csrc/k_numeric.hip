@@ -479,7 +479,9 @@ constexpr int kLongBS = 1024;
 // Bin 7: rows above 4096 entries, bitmap-rank windows.  The columns are
 // split into windows of WB = 2^17; per window:
 //  1. mark: every product of the window sets its column's bit in an LDS
-//     bitmap (read first, atomic OR only for a clear bit);
+//     bitmap (read first, atomic OR only for a clear bit).  A row whose
+//     column bitmap the symbolic phase kept (ls.bm) reads the window's
+//     words from it instead and walks its products only in step 3;
 //  2. rank: a scan of the per-word popcounts (DPP inside each wave, wave
 //     totals through LDS) gives each word the number of the row's columns
 //     before it, so the position of column c in the window's sorted output
@@ -531,6 +533,16 @@ struct StoredRange {
     }
 };
 
+// The whole B row (a single column window).
+struct RowRange {
+    const int *__restrict__ brpt;
+    __device__ __forceinline__ void operator()(int /*e*/, int k, int &bs, int &len) const
+    {
+        bs = brpt[k];
+        len = brpt[k + 1] - bs;
+    }
+};
+
 template <typename V>
 __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const V *__restrict__ aval, const int *__restrict__ brpt,
@@ -569,6 +581,10 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
         const int r = rows ? rows[ridx] : ridx;
         const int a_beg = arpt[r], a_end = arpt[r + 1];
+        // A row whose column bitmap the symbolic phase kept skips step 1:
+        // each window's words are read from it instead.
+        const int slot = ls.bm ? ls.bm_slot[r] : -1;
+        const unsigned *const sbm = slot >= 0 ? ls.bm + static_cast<long long>(slot) * ls.bm_words : nullptr;
         long long out = crpt[r];
         const long long row_end = crpt[r + 1];
         // First window: the one holding the smallest column of the row.
@@ -595,7 +611,9 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             int sn = 0, stot = 0;
             bool one = false;
             const int wi = w0 / R::WB;
-            {
+            if (sbm) {
+                nxt = last ? INT_MAX : wend;
+            } else {
                 RankMark<V> op{bcol, br, w0};
                 if (ls.wcut) {
                     const WinCutRange range{ls.wcut, bcol, cstride, wi, &nxt};
@@ -617,7 +635,16 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             int wtot = 0;
 #pragma unroll
             for (int i = 0; i < RPW; i++) {
-                cnt[i] = __popc(br[wave * WPW + i * kWave + lane].x);
+                const int wd = wave * WPW + i * kWave + lane;
+                unsigned x;
+                if (sbm) {
+                    const int g = wi * R::WORDS + wd;
+                    x = g < ls.bm_words ? sbm[g] : 0u;
+                    br[wd].x = x;
+                } else {
+                    x = br[wd].x;
+                }
+                cnt[i] = __popc(x);
                 inc[i] = wave_incl_scan(cnt[i]);
                 rsum[i] = __builtin_amdgcn_readlane(inc[i], kWave - 1);
                 wtot += rsum[i];
@@ -672,6 +699,10 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                         if (stot > 0) block_walk<kLongBS, true, V>(s_bst, sn, stot, op);
                     } else if (ls.wcut) {
                         const WinCutRange range{ls.wcut, bcol, cstride, wi, nullptr};
+                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                                                                 stot);
+                    } else if (sbm) {
+                        const RowRange range{brpt};
                         block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     } else {

@@ -187,10 +187,11 @@ struct WindowRange {
 __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const int *__restrict__ brpt, const int *__restrict__ bcol,
                                                       const int *__restrict__ rows, int nrows, int ncols,
-                                                      int wcols, int *__restrict__ row_nnz)
+                                                      int wcols, int *__restrict__ row_nnz, SymSave save)
 {
     extern __shared__ unsigned bm[];
     __shared__ int s_red[kLongBS / kWave];
+    __shared__ int s_slot;
     __shared__ BlockStage<kLongBS, double> s_bst;
     const int tid = threadIdx.x;
     // The bitmap is cleared once here and then by each window's count pass.
@@ -210,13 +211,42 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
             block_for_each_product<kLongBS, false, double>(a_beg, a_end, acol, static_cast<const double *>(nullptr),
                                                            range, op, s_bst, sn, stot);
             __syncthreads();
-            for (int i = tid; i < nwords; i += kLongBS) {
-                cnt += __popc(bm[i]);
-                bm[i] = 0u;
+            if (save.bm) {
+                // One window (the host keeps bitmaps only then): a row long
+                // enough for the numeric long-row kernel leaves its bitmap
+                // in global memory, copied out while it is cleared.
+                int c = 0;
+                for (int i = tid; i < nwords; i += kLongBS) c += __popc(bm[i]);
+                const int tot = block_sum<kLongBS>(c, s_red);
+                if (tid == 0) {
+                    int sl = -1;
+                    if (tot > save.min_nnz) {
+                        sl = atomicAdd(save.count, 1);
+                        if (sl < save.cap) save.slot[r] = sl;
+                        else sl = -1;
+                    }
+                    s_slot = sl;
+                }
+                __syncthreads();
+                if (s_slot >= 0) {
+                    unsigned *dst = save.bm + static_cast<size_t>(s_slot) * nwords;
+                    for (int i = tid; i < nwords; i += kLongBS) {
+                        dst[i] = bm[i];
+                        bm[i] = 0u;
+                    }
+                } else {
+                    for (int i = tid; i < nwords; i += kLongBS) bm[i] = 0u;
+                }
+                if (tid == 0) cnt = tot;
+            } else {
+                for (int i = tid; i < nwords; i += kLongBS) {
+                    cnt += __popc(bm[i]);
+                    bm[i] = 0u;
+                }
             }
             __syncthreads();
         }
-        const int tot = block_sum<kLongBS>(cnt, s_red);
+        const int tot = save.bm ? cnt : block_sum<kLongBS>(cnt, s_red);
         if (tid == 0) row_nnz[r] = tot;
     }
 }
@@ -259,7 +289,7 @@ void symbolic_init_attributes()
 }
 
 void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash,
-                         bool bitmap, hipStream_t s)
+                         bool bitmap, const SymSave &save, hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
@@ -276,7 +306,8 @@ void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *row
             const int words = static_cast<int>(std::min<long long>(kBitmapWords, (static_cast<long long>(B.N) + 31) / 32));
             const int wcols = words * 32;
             k_sym_long<<<nrows, kLongBS, sizeof(unsigned) * static_cast<size_t>(words), s>>>(
-                A.rpt, A.col, B.rpt, B.col, rows, nrows, B.N, wcols, row_nnz);
+                A.rpt, A.col, B.rpt, B.col, rows, nrows, B.N, wcols, row_nnz,
+                B.N <= kSymLongWindow ? save : SymSave{});
             break;
         }
     }

@@ -100,11 +100,25 @@ void launch_narrow_rpt(const long long *crpt64, int M, int *crpt32, hipStream_t 
 // --- symbolic (k_symbolic.hip) ---
 // Columns per bitmap window of the long-row symbolic kernel (128 KB of LDS).
 constexpr int kSymLongWindow = 1 << 20;
+
+// Column bitmaps the long-row symbolic kernel keeps for the numeric phase
+// (B's columns in one window): a row whose count exceeds min_nnz takes slot
+// atomicAdd(count, 1) of bm (words per slot: ceil(B.N / 32)) and, if that
+// slot is below cap, records it in slot[row] (left at -1 otherwise);
+// bm == nullptr keeps none.
+struct SymSave {
+    unsigned *bm = nullptr;
+    int *slot = nullptr;
+    int *count = nullptr;
+    int min_nnz = 0;
+    int cap = 0;
+};
 // Computes row_nnz for the rows of one bin.  rows == nullptr means rows 0..nrows-1.
 // hash: family for the short-row bins 0-4 (kHashLin or kHashFib in device.hpp).
 // bitmap: bins 0-4 count rows with a narrow column span in a bitmap.
+// save: bitmaps of bin 7 kept for the numeric phase (only when B.N <= kSymLongWindow).
 void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash,
-                         bool bitmap, hipStream_t s);
+                         bool bitmap, const SymSave &save, hipStream_t s);
 // Bytes of dynamic LDS the long-row symbolic kernel asks for.
 size_t symbolic_long_lds();
 
@@ -125,6 +139,12 @@ struct LongScratch {
     int span = 0;               // highest A-nonzero index + 1
     const int *wcut = nullptr;  // window cuts of every B row, or nullptr
     int nwin = 0;               // column windows of kRankWindow (wcut has nwin + 1 per B row)
+    // Column bitmaps kept by the symbolic phase (SymSave): row r's bitmap is
+    // bm + bm_slot[r] * bm_words when bm_slot[r] >= 0.  Only set when every
+    // window's B ranges come without a mark walk (wcut, or a single window).
+    const unsigned *bm = nullptr;
+    const int *bm_slot = nullptr;
+    int bm_words = 0;
 };
 
 // wcut[k * (nwin + 1) + w] = first entry of B row k with column >= w *

@@ -87,7 +87,8 @@ struct ops_workspace {
     hipEvent_t ev_k0[kNumBins] = {}, ev_k1[kNumBins] = {};
     DevStats *d_stats = nullptr;
     DevStats *h_stats = nullptr;  // pinned
-    DevBuf<int> row_flop, row_nnz, bins, cursor, spill, wcut, crpt32_tmp;
+    DevBuf<int> row_flop, row_nnz, bins, cursor, spill, wcut, crpt32_tmp, bm_slot;
+    DevBuf<unsigned> bm;  // column bitmaps kept by the symbolic phase
     DevBuf<long long> block_sums, crpt64;
     BinBounds bounds = default_bounds();
     int short_hash = kHashLin;
@@ -183,10 +184,32 @@ void read_kernel_ms(ops_workspace *ws, const int *size, double *ms)
     }
 }
 
+// Column windows of the long-row numeric kernel over B's columns.
+int rank_windows(int n) { return static_cast<int>((static_cast<long long>(n) + kRankWindow - 1) / kRankWindow); }
+
+// Whether the long-row numeric kernel gets a window-cut table of B: more
+// than one window and at most kMaxCutBytes.  OPSAMD_CUT_TABLE=0 turns the
+// table off (galloping path), for tests.
+bool cut_table_wanted(int bm, int bn)
+{
+    const int nwin = rank_windows(bn);
+    const long long bytes = static_cast<long long>(sizeof(int)) * bm * (nwin + 1);
+    const char *env = std::getenv("OPSAMD_CUT_TABLE");
+    const bool off = env && env[0] == '0';
+    return nwin > 1 && bytes <= kMaxCutBytes && !off;
+}
+
+// Most bytes of column bitmaps the symbolic phase keeps for the numeric
+// phase (further long rows are marked again by the numeric kernel).
+constexpr long long kMaxKeptBitmapBytes = 64ll << 30;
+
 // Setup, symbolic and scan phases.  crpt64_dst: where C's 64-bit row
 // offsets go; crpt32_dst (optional): int32 copy, written only if nnz fits.
+// keep_bitmaps: long rows may leave their column bitmaps in the workspace
+// for the numeric phase (pc.ls.bm); plans, whose numeric phase runs later
+// and repeatedly, pass false.
 void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore &pc, long long *crpt64_dst,
-                    int *crpt32_dst, int *bins_dst, hipStream_t s, ops_timings *t)
+                    int *crpt32_dst, int *bins_dst, hipStream_t s, ops_timings *t, bool keep_bitmaps)
 {
     const bool sep = t && t->measure_separate;
     const bool ev = t && t->kernel_events;
@@ -249,10 +272,40 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
         t->symbolic_binning = now_s() - t0;
     }
 
+    // Bitmaps of rows bound for the long-row numeric kernel are kept when
+    // B's columns fit one symbolic window and every numeric window's B
+    // ranges are known without a mark walk (one window, or the cut table).
+    SymSave save;
+    const int long_rows = st.sym_size[kNumBins - 1];
+    if (keep_bitmaps && long_rows > 0 && B.N <= kSymLongWindow &&
+        (rank_windows(B.N) == 1 || cut_table_wanted(B.M, B.N))) {
+        const long long words = (static_cast<long long>(B.N) + 31) / 32;
+        size_t free_b = 0, total_b = 0;
+        CK(hipMemGetInfo(&free_b, &total_b));
+        // The bitmaps this workspace already holds count as free.
+        const long long avail = static_cast<long long>(free_b + ws->bm.cap * sizeof(unsigned));
+        const long long budget = std::min<long long>(kMaxKeptBitmapBytes, avail / 4);
+        long long cap = std::min<long long>(long_rows, budget / (4 * words));
+        // OPSAMD_KEPT_BITMAPS=n keeps at most n (0: none), for tests.
+        const char *env = std::getenv("OPSAMD_KEPT_BITMAPS");
+        if (env && env[0]) cap = std::min(cap, std::atoll(env));
+        if (cap > 0) {
+            ws->bm.ensure(static_cast<size_t>(cap * words));
+            ws->bm_slot.ensure(static_cast<size_t>(M) + 1);
+            // slot = -1 for every row, count (at [M]) = 0
+            CK(hipMemsetAsync(ws->bm_slot.p, 0xFF, sizeof(int) * static_cast<size_t>(M), s));
+            CK(hipMemsetAsync(ws->bm_slot.p + M, 0, sizeof(int), s));
+            save = SymSave{ws->bm.p, ws->bm_slot.p, ws->bm_slot.p + M, bb.num[kNumBins - 2], static_cast<int>(cap)};
+            pc.ls.bm = ws->bm.p;
+            pc.ls.bm_slot = ws->bm_slot.p;
+            pc.ls.bm_words = static_cast<int>(words);
+        }
+    }
+
     t0 = now_s();
     dispatch_bins(ws, s, st.sym_size, ev, [&](int b, hipStream_t ps) {
         launch_symbolic_bin(b, A, B, sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b], ws->row_nnz.p,
-                            pc.hash, ws->sym_bitmap, ps);
+                            pc.hash, ws->sym_bitmap, save, ps);
     });
     if (sep) {
         CK(hipStreamSynchronize(s));
@@ -359,17 +412,19 @@ void numeric_scratch(PlanCore &pc, const ops_csr *A, const ops_csr *B, DevBuf<in
         pc.ls.span = static_cast<int>(a_index_span(A, s));
         cursor.ensure(4 * static_cast<size_t>(pc.ls.span) + 4);
         pc.ls.cursor = cursor.p;
-        const int nwin = static_cast<int>((static_cast<long long>(B->N) + kRankWindow - 1) / kRankWindow);
-        const long long bytes = static_cast<long long>(sizeof(int)) * B->M * (nwin + 1);
-        // OPSAMD_CUT_TABLE=0 turns the table off (galloping path), for tests.
-        const char *env = std::getenv("OPSAMD_CUT_TABLE");
-        const bool off = env && env[0] == '0';
-        if (nwin > 1 && bytes <= kMaxCutBytes && !off) {
+        const int nwin = rank_windows(B->N);
+        if (cut_table_wanted(B->M, B->N)) {
             wcut.ensure(static_cast<size_t>(B->M) * (nwin + 1));
             launch_window_cuts(as_in(B), nwin, wcut.p, s);
             CK(hipGetLastError());
             pc.ls.wcut = wcut.p;
             pc.ls.nwin = nwin;
+        }
+        // Kept bitmaps need a window's B ranges without a mark walk.
+        if (!pc.ls.wcut && nwin > 1) {
+            pc.ls.bm = nullptr;
+            pc.ls.bm_slot = nullptr;
+            pc.ls.bm_words = 0;
         }
     }
     if (pc.num_size[kSpillBin] > 0) {
@@ -423,7 +478,7 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
     }
     const double t_alloc_rpt = now_s() - t0;
 
-    symbolic_phase(ws, a, b, pc, crpt64_dst, crpt32_dst, ws->bins.p, s, t);
+    symbolic_phase(ws, a, b, pc, crpt64_dst, crpt32_dst, ws->bins.p, s, t, true);
     if (t) t->setup += t_alloc_rpt;
 
     if (C32 && pc.nnz > static_cast<long long>(INT32_MAX))
@@ -572,6 +627,8 @@ int ops_workspace_destroy(ops_workspace *ws)
     ws->cursor.release();
     ws->spill.release();
     ws->wcut.release();
+    ws->bm.release();
+    ws->bm_slot.release();
     ws->crpt32_tmp.release();
     ws->block_sums.release();
     ws->crpt64.release();
@@ -755,7 +812,7 @@ int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_p
         const int M = A->M;
         p->crpt64.ensure(static_cast<size_t>(M) + 1);
         p->bins.ensure(static_cast<size_t>(M) + 1);
-        symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr);
+        symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr, false);
         numeric_scratch(p->core, A, B, p->cursor, p->spill, p->wcut, s);
         CK(hipStreamSynchronize(s));
         if (nnz_c) *nnz_c = p->core.nnz;

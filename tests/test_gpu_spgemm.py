@@ -286,6 +286,27 @@ def test_long_rows_many_rank_subwindows(gpu):
     run(A, B, wide=True)
 
 
+@pytest.mark.parametrize("N", [90000, 200000])
+def test_long_rows_kept_bitmaps(gpu, monkeypatch, N):
+    """Long rows whose column bitmaps the symbolic phase keeps for the
+    numeric phase (B within 2^20 columns): all rows kept, none kept (every
+    window marked again) and only two kept (the rest marked), over one
+    bitmap window (whole B rows) and two (window-cut table), with one, two
+    and more value sub-windows per window."""
+    rng = np.random.default_rng(18)
+    K = 1000
+    lens = np.r_[rng.integers(350, 450, 3), rng.integers(120, 160, 3), rng.integers(40, 60, 3),
+                 rng.integers(1, 3, 2)]
+    A = random_csr(len(lens), K, lens, rng)
+    B = random_csr(K, N, rng.integers(250, 350, K), rng)
+    for kept in ("", "0", "2"):
+        if kept:
+            monkeypatch.setenv("OPSAMD_KEPT_BITMAPS", kept)
+        log(f"kept bitmaps N={N} limit={kept or 'none'}")
+        run(A, B)
+        run(A, B, np.float32)
+
+
 def test_long_rows_multi_window_cut_table_and_gallop(gpu, monkeypatch):
     """Long rows whose columns lie in blocks 0, 2 and 5 of six 2^18-column
     blocks: the bitmap windows (2^17 columns) in between are empty and
