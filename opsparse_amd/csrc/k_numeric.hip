@@ -602,6 +602,11 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
         int w0 = mn == INT_MAX ? ncols : (mn / R::WB) * R::WB;
         bool first = true;
         int p = 0;
+        // Kept bitmap words of the next window, loaded during this window's
+        // accumulate pass (the rank scan of a kept row otherwise waits for
+        // its load).
+        unsigned pf[RPW];
+        bool pf_ok = false;
         while (w0 < ncols) {
             const int wend = (ncols - w0 <= R::WB) ? ncols : w0 + R::WB;
             const bool last = wend >= ncols;
@@ -633,13 +638,15 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             // LDS reads); ranks come from DPP scans of the words' bit counts.
             int cnt[RPW], inc[RPW], rsum[RPW];
             int wtot = 0;
+            const bool use_pf = pf_ok;
+            pf_ok = false;
 #pragma unroll
             for (int i = 0; i < RPW; i++) {
                 const int wd = wave * WPW + i * kWave + lane;
                 unsigned x;
                 if (sbm) {
                     const int g = wi * R::WORDS + wd;
-                    x = g < ls.bm_words ? sbm[g] : 0u;
+                    x = use_pf ? pf[i] : (g < ls.bm_words ? sbm[g] : 0u);
                     br[wd].x = x;
                 } else {
                     x = br[wd].x;
@@ -689,6 +696,15 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 for (int i = tid; i < r1 - r0; i += kLongBS) vals[i] = V(0);
                 __syncthreads();
                 drain_vmem();
+                if (sbm && sub == 0 && !last) {
+                    // Kept rows visit every window in turn: the next is wi + 1.
+#pragma unroll
+                    for (int i = 0; i < RPW; i++) {
+                        const int g = (wi + 1) * R::WORDS + wave * WPW + i * kWave + lane;
+                        pf[i] = g < ls.bm_words ? sbm[g] : 0u;
+                    }
+                    pf_ok = true;
+                }
                 RankAcc<V> op{bcol, bval, br, vals, w0, r0, static_cast<unsigned>(r1 - r0)};
                 if (tot <= 2 * R::VCAP) {
                     // At most two sub-windows: walk the window's products over
