@@ -133,7 +133,8 @@ int ops_csr64_free(ops_csr64 *C);
  * Releasing synchronises the device, as hipFree does.  Library-allocated
  * arrays must be released through these calls, not hipFree; ops_free passes
  * any other pointer to hipFree.  The cache keeps at most `bytes` per device
- * (bytes < 0, the default: half of the device's memory; 0: no caching).
+ * (bytes < 0, the default: half of the device's memory; 0: no caching),
+ * freeing the least recently released blocks to make room for a new one.
  */
 typedef struct {
     int64_t cached_bytes, cached_blocks;  /* held by the cache */

@@ -8,8 +8,11 @@
 // request is served by the smallest cached block of at least the rounded
 // size and at most 2x it (small) or 1.25x it (64 MiB and above).  Cached
 // bytes per device are capped (default: half of the device's memory,
-// ops_mem_cache_set_limit changes it, 0 turns caching off); a released block
-// that does not fit under the cap is freed at once.
+// ops_mem_cache_set_limit changes it, 0 turns caching off).  A released
+// block within the cap is kept and the least recently released cached
+// blocks are freed to make room for it (so a caller alternating between two
+// products that do not fit together keeps the one it is using); a block
+// larger than the cap is freed at once.
 #include "outmem.hpp"
 
 #include <algorithm>
@@ -35,6 +38,10 @@ struct State {
     std::mutex mu;
     std::unordered_map<void *, Rec> live;        // handed out by out_malloc
     std::multimap<size_t, void *> pool[kMaxDev];  // cached blocks by size
+    // cached blocks by release order (oldest first) and each block's key there
+    std::map<unsigned long long, std::pair<size_t, void *>> age[kMaxDev];
+    std::unordered_map<void *, unsigned long long> age_of;
+    unsigned long long tick = 0;
     size_t cached[kMaxDev] = {};
     size_t live_bytes = 0;
     long long limit = -1;  // < 0: half of the device's memory
@@ -72,11 +79,53 @@ size_t limit_for(State &s, int dev)
     return s.dev_total[dev] / 2;
 }
 
+// Caller holds the lock.  Adds a released block to the cache of `dev`.
+void pool_put(State &s, int dev, size_t bytes, void *p)
+{
+    s.pool[dev].emplace(bytes, p);
+    const unsigned long long t = ++s.tick;
+    s.age[dev].emplace(t, std::make_pair(bytes, p));
+    s.age_of[p] = t;
+    s.cached[dev] += bytes;
+}
+
+// Caller holds the lock.  Removes cached block `it` of `dev` (not freed).
+void pool_take(State &s, int dev, std::multimap<size_t, void *>::iterator it)
+{
+    auto a = s.age_of.find(it->second);
+    if (a != s.age_of.end()) {
+        s.age[dev].erase(a->second);
+        s.age_of.erase(a);
+    }
+    s.cached[dev] -= it->first;
+    s.pool[dev].erase(it);
+}
+
+// Caller holds the lock; the device is current.  Frees the least recently
+// released cached block of `dev`.
+void evict_oldest(State &s, int dev)
+{
+    auto o = s.age[dev].begin();
+    const size_t bytes = o->second.first;
+    void *p = o->second.second;
+    auto range = s.pool[dev].equal_range(bytes);
+    for (auto it = range.first; it != range.second; ++it)
+        if (it->second == p) {
+            pool_take(s, dev, it);
+            break;
+        }
+    (void)hipFree(p);
+}
+
 // Frees every cached block of `dev` (the device must be current).
 void drain(State &s, int dev)
 {
-    for (auto &kv : s.pool[dev]) (void)hipFree(kv.second);
+    for (auto &kv : s.pool[dev]) {
+        s.age_of.erase(kv.second);
+        (void)hipFree(kv.second);
+    }
     s.pool[dev].clear();
+    s.age[dev].clear();
     s.cached[dev] = 0;
 }
 
@@ -98,8 +147,7 @@ hipError_t out_malloc(void **p, size_t bytes)
         if (it != pl.end() && it->first <= max_fit(need)) {
             *p = it->second;
             const size_t sz = it->first;
-            pl.erase(it);
-            s.cached[dev] -= sz;
+            pool_take(s, dev, it);
             s.live[*p] = Rec{sz, dev};
             s.live_bytes += sz;
             s.hits++;
@@ -160,9 +208,10 @@ hipError_t out_free(void *p)
     bool keep = false;
     if (e == hipSuccess) {
         std::lock_guard<std::mutex> g(s.mu);
-        if (s.cached[r.dev] + r.bytes <= limit_for(s, r.dev)) {
-            s.pool[r.dev].emplace(r.bytes, p);
-            s.cached[r.dev] += r.bytes;
+        const size_t cap = limit_for(s, r.dev);
+        if (r.bytes <= cap) {
+            while (s.cached[r.dev] + r.bytes > cap && !s.age[r.dev].empty()) evict_oldest(s, r.dev);
+            pool_put(s, r.dev, r.bytes, p);
             keep = true;
         }
     }
@@ -202,13 +251,8 @@ int ops_mem_cache_set_limit(int64_t bytes)
         if (s.pool[d].empty()) continue;
         if (d != cur) (void)hipSetDevice(d);
         const size_t cap = limit_for(s, d);
-        // Free the largest blocks until the device is under its cap.
-        while (s.cached[d] > cap && !s.pool[d].empty()) {
-            auto it = std::prev(s.pool[d].end());
-            (void)hipFree(it->second);
-            s.cached[d] -= it->first;
-            s.pool[d].erase(it);
-        }
+        // Free the least recently released blocks until the device is under its cap.
+        while (s.cached[d] > cap && !s.age[d].empty()) evict_oldest(s, d);
         if (d != cur) (void)hipSetDevice(cur);
     }
     return OPS_OK;

@@ -272,6 +272,36 @@ def test_output_cache_reuse_and_limit(gpu):
     mem_cache_release()
 
 
+def test_output_cache_keeps_recent_blocks(gpu):
+    """With a cache limit that holds either of two products' C but not both,
+    releasing one product's arrays evicts the other's (least recently
+    released) blocks, so repeating the product in use hits the cache."""
+    A1, A2 = G.rmat(12, 10, 8), G.rmat(11, 8, 5)
+    d1, d2 = DeviceCSR.from_host(A1, torch.float64), DeviceCSR.from_host(A2, torch.float64)
+    set_mem_cache_limit(-1)
+    mem_cache_release()
+    spgemm(d1, d1).free()
+    b1 = mem_cache_stats()["cached_bytes"]
+    mem_cache_release()
+    spgemm(d2, d2).free()
+    b2 = mem_cache_stats()["cached_bytes"]
+    assert b1 > b2 > 0
+    limit = b1 + b2 // 2
+    set_mem_cache_limit(limit)
+    spgemm(d2, d2).free()
+    spgemm(d1, d1).free()  # misses; releasing it evicts the blocks of the first product
+    st = mem_cache_stats()
+    assert st["cached_bytes"] <= limit
+    C = spgemm(d1, d1)
+    rpt, col, val = C.to_host()
+    oracle.compare(rpt, col, val, oracle.spgemm(A1, A1), TOL[np.float64])
+    C.free()
+    assert mem_cache_stats()["hits"] - st["hits"] >= 3  # rpt, col and val
+    assert mem_cache_stats()["cached_bytes"] <= limit
+    set_mem_cache_limit(-1)
+    mem_cache_release()
+
+
 def test_long_rows_many_rank_subwindows(gpu):
     """Rows of ~90K entries over two 2^17-column bitmap windows: the
     accumulate pass runs over several value sub-windows per window (12288
