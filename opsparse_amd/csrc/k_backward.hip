@@ -36,21 +36,21 @@ __device__ __forceinline__ void global_add(float *p, float v) { unsafeAtomicAdd(
 // enough (da starts at zero).
 template <int U, typename V>
 struct DaReduce {
-    V *sda;
+    Acc *sda;
     V *__restrict__ da;
-    __device__ __forceinline__ void add(int q, V v) { atomicAdd(sda + q, v); }
+    __device__ __forceinline__ void add(int q, Acc v) { atomicAdd(sda + q, v); }
     __device__ __forceinline__ void flush(int e, bool valid, bool shared)
     {
         wave_lds_sync();
         const int gl = lane_id() & (U - 1);
-        const V v = sda[gl];
-        sda[gl] = V(0);
+        const Acc v = sda[gl];
+        sda[gl] = Acc(0);
         wave_lds_sync();
-        if (valid && v != V(0)) {
+        if (valid && v != Acc(0)) {
             if (shared)
-                global_add(da + e, v);
+                global_add(da + e, static_cast<V>(v));
             else
-                da[e] += v;
+                da[e] += static_cast<V>(v);
         }
     }
 };
@@ -75,7 +75,7 @@ struct BwdHash {
         const int h = hash_find<T, H>(kt, it.c);
         const V g = h >= 0 ? vt[h] : V(0);
         global_add(db + it.k, a * g);
-        red.add(q, g * it.b);
+        red.add(q, static_cast<Acc>(g) * static_cast<Acc>(it.b));
     }
     __device__ __forceinline__ void chunk_end(int e, bool valid, bool shared) { red.flush(e, valid, shared); }
 };
@@ -96,14 +96,14 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
     constexpr int H = G <= kWave ? kHashLin : kHashFib;
     extern __shared__ unsigned long long smem_u64[];
     V *vals = reinterpret_cast<V *>(smem_u64);
-    V *sda = vals + RPB * T;
+    Acc *sda = reinterpret_cast<Acc *>(vals + RPB * T);  // RPB * T is even: 8-byte aligned
     int *keys = reinterpret_cast<int *>(sda + BS);
 
     const int tid = threadIdx.x;
     const int slot = tid / G;
     const int unit = (tid % G) / U;
     for (int i = tid; i < RPB * T; i += BS) keys[i] = -1;
-    sda[tid] = V(0);
+    sda[tid] = Acc(0);
     __syncthreads();
     const int ridx = blockIdx.x * RPB + slot;
     int r = 0;
@@ -173,7 +173,7 @@ struct BwdRank {
     int r0;
     int base;
     V *__restrict__ db;
-    V *sda;
+    Acc *sda;
     struct Item {
         int k;
         int c;
@@ -186,7 +186,7 @@ struct BwdRank {
         const int c = it.c - w0;
         const V g = gv[static_cast<int>(e.y) + __popc(e.x & ((1u << (c & 31)) - 1u)) - r0];
         global_add(db + it.k, a * g);
-        atomicAdd(sda + (q - base), g * it.b);
+        atomicAdd(sda + (q - base), static_cast<Acc>(g) * static_cast<Acc>(it.b));
     }
 };
 
@@ -216,13 +216,13 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
     extern __shared__ unsigned long long smem_u64[];
     uint2 *br = reinterpret_cast<uint2 *>(smem_u64);
     V *gv = reinterpret_cast<V *>(br + R::WORDS);
-    __shared__ V sda[kLongBS];
+    __shared__ Acc sda[kLongBS];
     __shared__ int s_cnt[kLongBS / kWave];
     __shared__ BlockStage<kLongBS, V> s_bst;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int lane = lane_id();
-    sda[tid] = V(0);
+    sda[tid] = Acc(0);
     for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
     __syncthreads();
 
@@ -284,10 +284,10 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
                     if (total > 0) block_walk<kLongBS, true, V>(s_bst, n, total, op);
                     const int e = c + tid;
                     if (e < a_end) {
-                        const V v = sda[tid];
-                        if (v != V(0)) {
-                            da[e] += v;
-                            sda[tid] = V(0);
+                        const Acc v = sda[tid];
+                        if (v != Acc(0)) {
+                            da[e] += static_cast<V>(v);
+                            sda[tid] = Acc(0);
                         }
                     }
                 }
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
 template <int G, int T, int BS, typename V>
 constexpr size_t bwd_lds()
 {
-    return (sizeof(V) * ((BS / G) * T + BS)) + sizeof(int) * (BS / G) * T;
+    return sizeof(V) * (BS / G) * T + sizeof(Acc) * BS + sizeof(int) * (BS / G) * T;
 }
 
 template <int G, int T, int BS, typename V>

@@ -33,7 +33,7 @@ struct NumInsert {
     const int *__restrict__ bcol;
     const V *__restrict__ bval;
     int *kt;
-    V *vt;
+    Acc *vt;
     struct Item {
         int c;
         V b;
@@ -43,7 +43,7 @@ struct NumInsert {
     {
         bool fresh;
         const int h = hash_insert<T, H>(kt, it.c, fresh);
-        atomicAdd(vt + h, a * it.b);
+        atomicAdd(vt + h, static_cast<Acc>(a) * static_cast<Acc>(it.b));
     }
 };
 
@@ -63,21 +63,21 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     constexpr int K = NMAX / U;  // sorted entries per lane
     static_assert(G <= kWave && BS % G == 0 && K >= 1 && T >= 2 * NMAX, "bad config");
     __shared__ __align__(16) int keys[RPB * T];
-    __shared__ __align__(16) V vals[RPB * T];
+    __shared__ __align__(16) Acc vals[RPB * T];
     __shared__ int ckeys[RPB * NMAX];
 
     const int tid = threadIdx.x;
     const int slot = tid / G;
     const int gl = tid % G;
     for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(keys)[i] = make_int4(-1, -1, -1, -1);
-    for (int i = tid; i < RPB * T * static_cast<int>(sizeof(V)) / 16; i += BS)
+    for (int i = tid; i < RPB * T * static_cast<int>(sizeof(Acc)) / 16; i += BS)
         reinterpret_cast<int4 *>(vals)[i] = make_int4(0, 0, 0, 0);
     __syncthreads();
 
     const int ridx = xcd_tile() * RPB + slot;
     int r = 0;
     int *kt = keys + slot * T;
-    V *vt = vals + slot * T;
+    Acc *vt = vals + slot * T;
     if (ridx < nrows) {
         r = rows ? rows[ridx] : ridx;
         const FullRow range{brpt};
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
         if (j < n_out) {
             const int h = hash_find<T, H>(kt, sk[k]);
             ccol[off + j] = sk[k];
-            cval[off + j] = h >= 0 ? vt[h] : V(0);
+            cval[off + j] = h >= 0 ? static_cast<V>(vt[h]) : V(0);
         }
     }
 }
@@ -150,7 +150,7 @@ struct RankAcc {
     const int *__restrict__ bcol;
     const V *__restrict__ bval;
     const uint2 *br;
-    V *vals;
+    Acc *vals;
     int w0;
     int r0;
     unsigned nr;
@@ -160,7 +160,7 @@ struct RankAcc {
     };
     struct Post {
         unsigned at;  // >= nr: nothing to add
-        V v;
+        Acc v;
     };
     __device__ __forceinline__ Item load(int k) const { return Item{bcol[k], bval[k]}; }
     __device__ __forceinline__ uint2 prep(const Item &it) const { return br[(it.c - w0) >> 5]; }
@@ -168,7 +168,7 @@ struct RankAcc {
     {
         const int c = it.c - w0;
         const unsigned at = e.y + __popc(e.x & ((1u << (c & 31)) - 1u)) - static_cast<unsigned>(r0);
-        return Post{ok ? at : nr, a * it.b};
+        return Post{ok ? at : nr, static_cast<Acc>(a) * static_cast<Acc>(it.b)};
     }
     __device__ __forceinline__ void post(const Post &p)
     {
@@ -195,13 +195,13 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
     // a previous kernel of the same stream appended to a list).
     if (dcount) nrows = *dcount;
     // Hash path: T values | T keys | T/2 sorted keys.
-    V *vt = reinterpret_cast<V *>(smem_u64);
+    Acc *vt = reinterpret_cast<Acc *>(smem_u64);
     int *kt = reinterpret_cast<int *>(vt + T);
     int *sk = kt + T;
     // Rank path over the same memory: T bitmap words with ranks | T/2 values.
     uint2 *br = reinterpret_cast<uint2 *>(smem_u64);
-    V *rv = reinterpret_cast<V *>(br + T);
-    static_assert(sizeof(uint2) * T + sizeof(V) * (T / 2) <= sizeof(V) * T + sizeof(int) * (T + T / 2),
+    Acc *rv = reinterpret_cast<Acc *>(br + T);
+    static_assert(sizeof(uint2) * T + sizeof(Acc) * (T / 2) <= sizeof(Acc) * T + sizeof(int) * (T + T / 2),
                   "rank path must fit in the hash path's LDS");
     __shared__ int s_pos;
     __shared__ int s_scan[BS / kWave + 1];
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
                 if (w < nw) br[w].y = static_cast<unsigned>(run + inc[i] - cnt[i]);
                 run += __builtin_amdgcn_readlane(inc[i], kWave - 1);
             }
-            for (int i = tid; i < n; i += BS) rv[i] = V(0);
+            for (int i = tid; i < n; i += BS) rv[i] = Acc(0);
             __syncthreads();
             {
                 RankAcc<V> op{bcol, bval, br, rv, mn, 0, static_cast<unsigned>(n)};
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
                 }
             }
             __syncthreads();
-            for (int i = tid; i < n; i += BS) cval[off + i] = rv[i];
+            for (int i = tid; i < n; i += BS) cval[off + i] = static_cast<V>(rv[i]);
             __syncthreads();
             int *scol = reinterpret_cast<int *>(rv);
             for (int w = tid; w < nw; w += BS) {
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
         }
         for (int i = tid; i < T; i += BS) {
             kt[i] = -1;
-            vt[i] = V(0);
+            vt[i] = Acc(0);
         }
         if (tid == 0) s_pos = 0;
         __syncthreads();
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
             const int key = sk[j];
             const int h = hash_find<T, kHashFib>(kt, key);
             ccol[off + j] = key;
-            cval[off + j] = h >= 0 ? vt[h] : V(0);
+            cval[off + j] = h >= 0 ? static_cast<V>(vt[h]) : V(0);
         }
         __syncthreads();
     }
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
 template <int NMAX, int W, typename V>
 struct WaveRankLds {
     uint2 br[W];
-    V vals[NMAX];
+    Acc vals[NMAX];
 };
 
 template <int NMAX, int W, int WPB, typename V>
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
                                                            V *__restrict__ cval, int *__restrict__ spill)
 {
     constexpr int WPL = W / kWave;  // bitmap words per lane in the rank scan
-    static_assert(W % kWave == 0 && NMAX * sizeof(V) >= NMAX * sizeof(int), "layout");
+    static_assert(W % kWave == 0 && NMAX * sizeof(Acc) >= NMAX * sizeof(int), "layout");
     __shared__ WaveRankLds<NMAX, W, V> s_lds[WPB];
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     }
     const long long off = crpt[r];
     const int n = min(static_cast<int>(crpt[r + 1] - off), NMAX);
-    for (int i = lane; i < n; i += kWave) L.vals[i] = V(0);
+    for (int i = lane; i < n; i += kWave) L.vals[i] = Acc(0);
     wave_lds_sync();
     {
         RankAcc<V> op{bcol, bval, L.br, L.vals, mn, 0, static_cast<unsigned>(n)};
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
             for_each_product<kWave, true, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
     }
     wave_lds_sync();
-    for (int i = lane; i < n; i += kWave) cval[off + i] = L.vals[i];
+    for (int i = lane; i < n; i += kWave) cval[off + i] = static_cast<V>(L.vals[i]);
     wave_lds_sync();
     int *scol = reinterpret_cast<int *>(L.vals);
     for (int w = lane; w < nw; w += kWave) {
@@ -514,8 +514,8 @@ template <typename V>
 struct RankWin {
     static constexpr int WB = kRankWindow;                      // columns per bitmap window
     static constexpr int WORDS = WB / 32;                       // bitmap words (32 KB)
-    static constexpr int VCAP = sizeof(V) == 8 ? 12288 : 24576;  // values per accumulate pass (96 KB)
-    static constexpr size_t lds = sizeof(V) * VCAP + sizeof(uint2) * WORDS;
+    static constexpr int VCAP = 12288;                          // values per accumulate pass (96 KB)
+    static constexpr size_t lds = sizeof(Acc) * VCAP + sizeof(uint2) * WORDS;
 };
 
 // B range of an A-nonzero recorded by an earlier walk of the same window:
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     constexpr int RPW = WPW / kWave;
     static_assert(R::WORDS % kLongBS == 0, "words per thread");
     extern __shared__ unsigned long long smem_u64[];
-    V *vals = reinterpret_cast<V *>(smem_u64);
+    Acc *vals = reinterpret_cast<Acc *>(smem_u64);
     uint2 *br = reinterpret_cast<uint2 *>(vals + R::VCAP);
     __shared__ int s_scan[kLongBS / kWave + 1];
     __shared__ int s_min[kLongBS / kWave];
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     __syncthreads();
                     cb = s_cut;
                 }
-                for (int i = tid; i < r1 - r0; i += kLongBS) vals[i] = V(0);
+                for (int i = tid; i < r1 - r0; i += kLongBS) vals[i] = Acc(0);
                 __syncthreads();
                 drain_vmem();
                 if (sbm && sub == 0 && !last) {
@@ -748,7 +748,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 // coalesced.
                 const int nsub = r1 - r0;
                 const long long lim = min(static_cast<long long>(nsub), row_end - (out + r0));
-                for (int i = tid; i < lim; i += kLongBS) cval[out + r0 + i] = vals[i];
+                for (int i = tid; i < lim; i += kLongBS) cval[out + r0 + i] = static_cast<V>(vals[i]);
                 __syncthreads();
                 int *scol = reinterpret_cast<int *>(vals);
                 const int wa = (ca - w0) >> 5;
@@ -803,7 +803,7 @@ void launch_small(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, co
 template <int T, int BS, typename V>
 constexpr size_t block_lds()
 {
-    return sizeof(V) * T + sizeof(int) * T + sizeof(int) * (T / 2);
+    return sizeof(Acc) * T + sizeof(int) * T + sizeof(int) * (T / 2);
 }
 
 template <int T, int BS, typename V>

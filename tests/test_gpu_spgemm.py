@@ -305,7 +305,8 @@ def test_output_cache_keeps_recent_blocks(gpu):
 def test_long_rows_many_rank_subwindows(gpu):
     """Rows of ~90K entries over two 2^17-column bitmap windows: the
     accumulate pass runs over several value sub-windows per window (12288
-    FP64 / 24576 FP32 entries each), cut at the column of each rank
+    entries each, FP64 accumulators for both value types), cut at the
+    column of each rank
     boundary."""
     rng = np.random.default_rng(12)
     N = 200000
