@@ -155,20 +155,27 @@ void dispatch_bins(ops_workspace *ws, hipStream_t s, const int *size, bool timin
     int used = 0;
     for (int b = 0; b < kNumBins; b++) used += size[b] > 0 ? 1 : 0;
     if (used == 0) return;
-    CK(hipEventRecord(ws->ev_fork, s));
-    // Largest bins first so the long kernels start earliest.
+    // The fork is recorded before the first launch, so the pool streams do
+    // not wait for the bin that runs on s itself.
+    if (used > 1) CK(hipEventRecord(ws->ev_fork, s));
+    // Largest bins first so the long kernels start earliest; the first one
+    // runs on the caller's stream (no fork or join for it).
     const int order[kNumBins] = {7, 6, 5, 4, 3, 2, 1, 0};
+    bool first = true;
     for (int oi = 0; oi < kNumBins; oi++) {
         const int b = order[oi];
         if (size[b] <= 0) continue;
-        hipStream_t ps = ws->pool[b];
-        CK(hipStreamWaitEvent(ps, ws->ev_fork, 0));
+        hipStream_t ps = first ? s : ws->pool[b];
+        if (!first) CK(hipStreamWaitEvent(ps, ws->ev_fork, 0));
         if (timing) CK(hipEventRecord(ws->ev_k0[b], ps));
         launch(b, ps);
         CK(hipGetLastError());
         if (timing) CK(hipEventRecord(ws->ev_k1[b], ps));
-        CK(hipEventRecord(ws->ev_join[b], ps));
-        CK(hipStreamWaitEvent(s, ws->ev_join[b], 0));
+        if (!first) {
+            CK(hipEventRecord(ws->ev_join[b], ps));
+            CK(hipStreamWaitEvent(s, ws->ev_join[b], 0));
+        }
+        first = false;
     }
 }
 
