@@ -843,8 +843,10 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows
         case 0: launch_small<16, 32, 16, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
         case 1: launch_small<32, 64, 32, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
         case 2: launch_small<64, 128, 64, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
-        case 3: launch_small<64, 256, 128, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
-        case 4: launch_small<64, 512, 256, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
+        // Bins 3-4: one row (one wave) per block, which measured 2-3% faster
+        // than four rows per block on the stencil and the batch.
+        case 3: launch_small<64, 256, 128, 64, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
+        case 4: launch_small<64, 512, 256, 64, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
         case kSpillBin:
             if (spill) {
                 // Wave per row; rows with a wide column span go to the block kernel.
