@@ -941,6 +941,80 @@ __device__ __forceinline__ void unit_bitonic_sort_keys(int (&key)[K])
     }
 }
 
+// Bitonic sort of BS*K int keys over a block.  Thread (wave w, lane l)
+// holds keys k = 0..K-1 at positions w*64*K + k*64 + l; on return position j
+// holds the j-th smallest key.  Strides below 64*K run in registers (key
+// swaps) and lane exchanges; only longer strides go through `lds` (BS*K
+// ints), between barriers: 18 barriers for 4096 keys instead of one per
+// network stage.  Every thread of the block calls it.
+template <int BS, int K>
+__device__ __forceinline__ void block_bitonic_sort_keys(int (&key)[K], int *lds)
+{
+    constexpr int WE = kWave * K;
+    constexpr int NEL = BS * K;
+    static_assert((K & (K - 1)) == 0 && (NEL & (NEL - 1)) == 0, "powers of two");
+    const int lane = lane_id();
+    const int base = static_cast<int>(threadIdx.x / kWave) * WE;
+#pragma unroll 1
+    for (int size = 2; size <= NEL; size <<= 1) {
+        if (size > WE) {
+            // strides size/2 .. WE cross waves
+#pragma unroll
+            for (int k = 0; k < K; k++) lds[base + k * kWave + lane] = key[k];
+            __syncthreads();
+#pragma unroll 1
+            for (int stride = size >> 1; stride >= WE; stride >>= 1) {
+#pragma unroll
+                for (int i0 = 0; i0 < NEL / 2; i0 += BS) {
+                    const int i = i0 + static_cast<int>(threadIdx.x);
+                    const int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));
+                    const int hi = lo + stride;
+                    const bool asc = (lo & size) == 0;
+                    const int a = lds[lo], b = lds[hi];
+                    if ((a > b) == asc) {
+                        lds[lo] = b;
+                        lds[hi] = a;
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) key[k] = lds[base + k * kWave + lane];
+            __syncthreads();
+        }
+        // strides below WE and at least 64: swaps between a lane's keys
+#pragma unroll 1
+        for (int stride = min(size >> 1, WE >> 1); stride >= kWave; stride >>= 1) {
+            const int ks = stride / kWave;
+#pragma unroll
+            for (int m = 1; m < K; m <<= 1) {
+                if (ks != m) continue;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if ((k ^ m) > k) {
+                        const bool asc = ((base + k * kWave) & size) == 0;
+                        const int a = key[k], b = key[k ^ m];
+                        key[k] = asc ? min(a, b) : max(a, b);
+                        key[k ^ m] = asc ? max(a, b) : min(a, b);
+                    }
+                }
+            }
+        }
+        // strides below 64: lane exchanges
+#pragma unroll 1
+        for (int stride = min(size >> 1, kWave >> 1); stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = base + k * kWave + lane;
+                const bool asc = (j & size) == 0;
+                const bool lower = (j & stride) == 0;
+                const int o = bperm(key[k], lane ^ stride);
+                key[k] = (asc == lower) ? min(key[k], o) : max(key[k], o);
+            }
+        }
+    }
+}
+
 // Block-wide exclusive scan of one int per thread; returns the exclusive
 // prefix and writes the block total.  `scratch` needs BS/64 + 1 ints of LDS.
 template <int BS>

@@ -180,8 +180,9 @@ struct RankAcc {
 // Rows whose columns span fewer than 32*T columns take the bitmap-rank path
 // of the long-row kernel in one window (a bitmap word per 32 columns from
 // the row's smallest column; no hash, no sort).  Other rows use a table of
-// T slots in dynamic LDS, then compaction of the keys and an LDS bitonic
-// sort of the keys; the values are read back from the table by key.
+// T slots in dynamic LDS, then compaction of the keys and a block bitonic
+// sort of the keys (in registers and lane exchanges, through LDS only for
+// strides across waves); the values are read back from the table by key.
 template <int T, int BS, typename V>
 __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                   const V *__restrict__ aval, const int *__restrict__ brpt,
@@ -208,6 +209,8 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
     __shared__ int s_mm[2 * (BS / kWave)];
     __shared__ BlockStage<BS, V> s_bst;
     constexpr int NU = BS / kWave;
+    constexpr int KPT = T / 2 / BS;  // sorted keys per thread on the hash path
+    static_assert(KPT >= 1 && KPT * BS == T / 2, "sort layout");
     constexpr int WPW = T / NU;  // bitmap words per wave in the rank scan
     constexpr int RPW = WPW / kWave;
     static_assert(T % (NU * kWave) == 0, "words per wave");
@@ -330,32 +333,26 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
             const int pos = base + __popcll(m & ((1ull << lane_id()) - 1ull));
             if (valid && pos < T / 2) sk[pos] = key;
         }
-        int p2 = 1;
-        while (p2 < n) p2 <<= 1;
         __syncthreads();
-        for (int i = n + tid; i < p2; i += BS) sk[i] = INT_MAX;
-        __syncthreads();
-        for (int size = 2; size <= p2; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int i = tid; i < (p2 >> 1); i += BS) {
-                    // pair index i -> (lo, lo + stride); stride is a power of two
-                    const int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));
-                    const int hi = lo + stride;
-                    const bool asc = (lo & size) == 0;
-                    const int a = sk[lo], b = sk[hi];
-                    if ((a > b) == asc) {
-                        sk[lo] = b;
-                        sk[hi] = a;
-                    }
-                }
-                __syncthreads();
-            }
+        // Sort the row's keys (padded to T/2 with INT_MAX) with KPT per
+        // thread in registers; key j then goes to output position j.
+        const int sbase = wave * kWave * KPT;
+        int key[KPT];
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int j = sbase + k * kWave + lane_id();
+            key[k] = j < n ? sk[j] : INT_MAX;
         }
-        for (int j = tid; j < n; j += BS) {
-            const int key = sk[j];
-            const int h = hash_find<T, kHashFib>(kt, key);
-            ccol[off + j] = key;
-            cval[off + j] = h >= 0 ? static_cast<V>(vt[h]) : V(0);
+        __syncthreads();
+        block_bitonic_sort_keys<BS, KPT>(key, sk);
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int j = sbase + k * kWave + lane_id();
+            if (j < n) {
+                const int h = hash_find<T, kHashFib>(kt, key[k]);
+                ccol[off + j] = key[k];
+                cval[off + j] = h >= 0 ? static_cast<V>(vt[h]) : V(0);
+            }
         }
         __syncthreads();
     }
