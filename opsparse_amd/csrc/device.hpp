@@ -853,56 +853,6 @@ __device__ __forceinline__ int hash_find(const int *tab, int key)
     return -1;
 }
 
-// Bitonic sort of K*U (key, value) pairs held K per lane across a unit of U
-// lanes; element j = k*U + (lane % U) ends up holding the j-th smallest key.
-// Keys must be distinct except for INT_MAX padding.
-template <int U, int K, typename V>
-__device__ __forceinline__ void unit_bitonic_sort(int (&key)[K], V (&val)[K])
-{
-    const int lane = lane_id();
-    const int gl = lane & (U - 1);
-    constexpr int NEL = K * U;
-#pragma unroll
-    for (int size = 2; size <= NEL; size <<= 1) {
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            if (stride >= U) {
-                const int ks = stride / U;
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const int pk = k ^ ks;
-                    if (pk > k) {
-                        const int j = k * U + gl;
-                        const bool asc = (j & size) == 0;
-                        if ((key[k] > key[pk]) == asc) {
-                            const int tk = key[k];
-                            key[k] = key[pk];
-                            key[pk] = tk;
-                            const V tv = val[k];
-                            val[k] = val[pk];
-                            val[pk] = tv;
-                        }
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const int j = k * U + gl;
-                    const bool asc = (j & size) == 0;
-                    const bool lower = (j & stride) == 0;
-                    const int ok = bperm(key[k], lane ^ stride);
-                    const V ov = bperm(val[k], lane ^ stride);
-                    const bool take_min = (asc == lower);
-                    if (take_min ? (ok < key[k]) : (ok > key[k])) {
-                        key[k] = ok;
-                        val[k] = ov;
-                    }
-                }
-            }
-        }
-    }
-}
-
 // Bitonic sort of K*U int keys held K per lane across a unit of U lanes;
 // element j = k*U + (lane % U) ends up holding the j-th smallest key.
 template <int U, int K>

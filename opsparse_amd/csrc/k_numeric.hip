@@ -12,7 +12,8 @@
 // the reference ranks every entry against every other entry, O(n^2)
 // (numeric.cuh:97-106, 194-204, 382-392); here rows of up to 256 entries
 // are compacted and bitonic-sorted in registers across their unit
-// (cross-lane ds_bpermute) and rows up to 4096 are bitonic-sorted in LDS.
+// (cross-lane ds_bpermute) and rows up to 4096 are bitonic-sorted across
+// the block (registers and lane exchanges, LDS for strides across waves).
 // Rows above 4096 entries use no hash and no sort: a bitmap of the row's
 // columns per 2^17-column window gives every column its rank (its position
 // in the sorted output), values accumulate at that rank in LDS, and the
