@@ -79,6 +79,17 @@ def test_uniform_4096_fp64_fp32(gpu):
     run(A, A, np.float32)
 
 
+@pytest.mark.parametrize("make", ["stencil", "rmat"])
+def test_fp32_entries_rounded_once(gpu, make):
+    """FP32 inputs are multiplied and summed in FP64 and each entry is
+    rounded to FP32 once, so against FP64 sums of the same FP32 inputs the
+    error is at most half an FP32 ulp: max |d| / sum |a*b| <= 2^-24 (an FP32
+    accumulation would exceed it on rows of many products)."""
+    A = G.stencil27(20, 1) if make == "stencil" else G.rmat(12, 16, 3)
+    e = run(A, A, np.float32)
+    assert e["max_norm"] <= 2.0 ** -24 * 1.0001, e
+
+
 def test_stencil_small(gpu):
     A = G.stencil27(24, 1)
     run(A, A)
