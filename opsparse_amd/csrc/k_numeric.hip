@@ -384,87 +384,124 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
     WaveRankLds<NMAX, W, V> &L = s_lds[wave];
-    const int ridx = xcd_tile() * WPB + wave;
-    if (ridx >= nrows) return;  // whole wave: nothing below synchronises the block
-    const int r = rows ? rows[ridx] : ridx;
-    const int a_beg = arpt[r], a_end = arpt[r + 1];
+    // Each wave takes two consecutive rows of the list.  The loads that
+    // start a row (row id, A row, B row offsets, first and last B columns:
+    // four dependent steps) are issued for both rows together, so the second
+    // row's chain overlaps the first's; the rows are then processed in turn.
+    const int ridx0 = (xcd_tile() * WPB + wave) * 2;
+    if (ridx0 >= nrows) return;  // whole wave: nothing below synchronises the block
+    struct Start {
+        int r, a_beg, a_end, cbs, clen, mn, mx;
+        V cav;
+    };
     // A row of at most 64 A-nonzeros is loaded once: lane l keeps its
     // A-nonzero's B range and value for the span check and both walks.
-    const bool one = a_end - a_beg <= kWave;
-    const int cnt = min(kWave, a_end - a_beg);
-    int cbs = 0, clen = 0;
-    V cav = V(0);
-    int mn = INT_MAX, mx = -1;
-    for (int e = a_beg + lane; e < a_end; e += kWave) {
-        const int k = acol[e];
-        const int b0 = brpt[k], b1 = brpt[k + 1];
-        if (one) {
-            cbs = b0;
-            clen = b1 - b0;
-            cav = aval[e];
+    auto row_start = [&](int ridx, Start &st) {
+        st.r = rows ? rows[ridx] : ridx;
+        st.a_beg = arpt[st.r];
+        st.a_end = arpt[st.r + 1];
+        st.cbs = 0;
+        st.clen = 0;
+        st.cav = V(0);
+        st.mn = INT_MAX;
+        st.mx = -1;
+        const int e = st.a_beg + lane;
+        if (e < st.a_end) {
+            const int k = acol[e];
+            const int b0 = brpt[k], b1 = brpt[k + 1];
+            st.cbs = b0;
+            st.clen = b1 - b0;
+            st.cav = aval[e];
+            if (b0 < b1) {
+                st.mn = bcol[b0];
+                st.mx = bcol[b1 - 1];
+            }
         }
-        if (b0 < b1) {
-            mn = min(mn, bcol[b0]);
-            mx = max(mx, bcol[b1 - 1]);
+    };
+    auto row_span_rest = [&](Start &st) {
+        for (int e = st.a_beg + kWave + lane; e < st.a_end; e += kWave) {
+            const int k = acol[e];
+            const int b0 = brpt[k], b1 = brpt[k + 1];
+            if (b0 < b1) {
+                st.mn = min(st.mn, bcol[b0]);
+                st.mx = max(st.mx, bcol[b1 - 1]);
+            }
         }
-    }
-    mn = unit_min<kWave>(mn);
-    mx = unit_max<kWave>(mx);
-    if (mx < 0) return;  // no products: the row is empty
-    if (mx - mn >= 32 * W) {
-        if (lane == 0) spill[1 + atomicAdd(spill, 1)] = r;
-        return;
-    }
-    const int nw = ((mx - mn) >> 5) + 1;
-    for (int i = lane; i < nw; i += kWave) L.br[i] = make_uint2(0u, 0u);
-    wave_lds_sync();
-    const FullRow range{brpt};
-    {
-        RankMark<V> op{bcol, L.br, mn};
-        if (one)
-            walk_chunk<kWave, false, V>(cnt, cbs, clen, cav, op, 0, 1);
-        else
-            for_each_product<kWave, false, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
-    }
-    wave_lds_sync();
-    // Ranks: lanes take 64 consecutive words at a time (conflict-free
-    // reads), DPP scans of the word popcounts.
-    int run = 0;
+        st.mn = unit_min<kWave>(st.mn);
+        st.mx = unit_max<kWave>(st.mx);
+    };
+    auto process = [&](const Start &st) {
+        const int r = st.r, a_beg = st.a_beg, a_end = st.a_end;
+        const int mn = st.mn, mx = st.mx;
+        const bool one = a_end - a_beg <= kWave;
+        const int cnt = min(kWave, a_end - a_beg);
+        const int cbs = st.cbs, clen = st.clen;
+        const V cav = st.cav;
+        if (mx < 0) return;  // no products: the row is empty
+        if (mx - mn >= 32 * W) {
+            if (lane == 0) spill[1 + atomicAdd(spill, 1)] = r;
+            return;
+        }
+        const int nw = ((mx - mn) >> 5) + 1;
+        for (int i = lane; i < nw; i += kWave) L.br[i] = make_uint2(0u, 0u);
+        wave_lds_sync();
+        const FullRow range{brpt};
+        {
+            RankMark<V> op{bcol, L.br, mn};
+            if (one)
+                walk_chunk<kWave, false, V>(cnt, cbs, clen, cav, op, 0, 1);
+            else
+                for_each_product<kWave, false, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
+        }
+        wave_lds_sync();
+        // Ranks: lanes take 64 consecutive words at a time (conflict-free
+        // reads), DPP scans of the word popcounts.
+        int run = 0;
 #pragma unroll
-    for (int i = 0; i < WPL; i++) {
-        const int w = i * kWave + lane;
-        const int c = w < nw ? __popc(L.br[w].x) : 0;
-        const int inc = wave_incl_scan(c);
-        if (w < nw) L.br[w].y = static_cast<unsigned>(run + inc - c);
-        run += __builtin_amdgcn_readlane(inc, kWave - 1);
-    }
-    const long long off = crpt[r];
-    const int n = min(static_cast<int>(crpt[r + 1] - off), NMAX);
-    for (int i = lane; i < n; i += kWave) L.vals[i] = Acc(0);
-    wave_lds_sync();
-    {
-        RankAcc<V> op{bcol, bval, L.br, L.vals, mn, 0, static_cast<unsigned>(n)};
-        if (one)
-            walk_chunk<kWave, true, V>(cnt, cbs, clen, cav, op, 0, 1);
-        else
-            for_each_product<kWave, true, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
-    }
-    wave_lds_sync();
-    for (int i = lane; i < n; i += kWave) cval[off + i] = static_cast<V>(L.vals[i]);
-    wave_lds_sync();
-    int *scol = reinterpret_cast<int *>(L.vals);
-    for (int w = lane; w < nw; w += kWave) {
-        const uint2 ew = L.br[w];
-        unsigned bits = ew.x;
-        while (bits) {
-            const int bit = __ffs(bits) - 1;
-            bits &= bits - 1u;
-            const int rr = static_cast<int>(ew.y) + __popc(ew.x & ((1u << bit) - 1u));
-            if (rr < n) scol[rr] = mn + w * 32 + bit;
+        for (int i = 0; i < WPL; i++) {
+            const int w = i * kWave + lane;
+            const int c = w < nw ? __popc(L.br[w].x) : 0;
+            const int inc = wave_incl_scan(c);
+            if (w < nw) L.br[w].y = static_cast<unsigned>(run + inc - c);
+            run += __builtin_amdgcn_readlane(inc, kWave - 1);
         }
-    }
-    wave_lds_sync();
-    for (int i = lane; i < n; i += kWave) ccol[off + i] = scol[i];
+        const long long off = crpt[r];
+        const int n = min(static_cast<int>(crpt[r + 1] - off), NMAX);
+        for (int i = lane; i < n; i += kWave) L.vals[i] = Acc(0);
+        wave_lds_sync();
+        {
+            RankAcc<V> op{bcol, bval, L.br, L.vals, mn, 0, static_cast<unsigned>(n)};
+            if (one)
+                walk_chunk<kWave, true, V>(cnt, cbs, clen, cav, op, 0, 1);
+            else
+                for_each_product<kWave, true, V>(a_beg, a_end, 0, 1, acol, aval, range, op);
+        }
+        wave_lds_sync();
+        for (int i = lane; i < n; i += kWave) cval[off + i] = static_cast<V>(L.vals[i]);
+        wave_lds_sync();
+        int *scol = reinterpret_cast<int *>(L.vals);
+        for (int w = lane; w < nw; w += kWave) {
+            const uint2 ew = L.br[w];
+            unsigned bits = ew.x;
+            while (bits) {
+                const int bit = __ffs(bits) - 1;
+                bits &= bits - 1u;
+                const int rr = static_cast<int>(ew.y) + __popc(ew.x & ((1u << bit) - 1u));
+                if (rr < n) scol[rr] = mn + w * 32 + bit;
+            }
+        }
+        wave_lds_sync();
+        for (int i = lane; i < n; i += kWave) ccol[off + i] = scol[i];
+        wave_lds_sync();  // the next row rewrites this wave's LDS
+    };
+    const bool two = ridx0 + 1 < nrows;
+    Start s0, s1;
+    row_start(ridx0, s0);
+    row_start(two ? ridx0 + 1 : ridx0, s1);
+    row_span_rest(s0);
+    row_span_rest(s1);
+    process(s0);
+    if (two) process(s1);
 }
 
 __global__ void k_zero_count(int *p)
@@ -850,7 +887,7 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows
                 // Wave per row; rows with a wide column span go to the block kernel.
                 constexpr int WPB = 4;
                 k_zero_count<<<1, kWave, 0, s>>>(spill);
-                k_num_wrank<1024, 512, WPB, V><<<xcd_grid((nrows + WPB - 1) / WPB), WPB * kWave, 0, s>>>(
+                k_num_wrank<1024, 512, WPB, V><<<xcd_grid((nrows + 2 * WPB - 1) / (2 * WPB)), WPB * kWave, 0, s>>>(
                     A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows,
                     nrows, crpt, ccol, cval, spill);
                 launch_block<2048, 256, V>(A, B, spill + 1, nrows, crpt, ccol, cval, s, spill, std::min(nrows, 4096));
