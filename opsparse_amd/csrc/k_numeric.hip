@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
                                                   const int *__restrict__ bcol, const V *__restrict__ bval,
                                                   const int *__restrict__ rows, int nrows,
                                                   const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                  V *__restrict__ cval)
+                                                  V *__restrict__ cval, bool pack)
 {
     constexpr int U = G;
     constexpr int RPB = BS / G;
@@ -89,7 +89,10 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     if (ridx >= nrows) return;
 
     // Compact the row's keys (slot order) into its key list; the table stays
-    // intact and supplies the values after the keys are sorted.
+    // intact and supplies the values after the keys are sorted.  With pack
+    // (every column below 2^(31 - log2 T)) a key carries its slot in its low
+    // bits, so the sorted list gives each value's slot without a lookup.
+    constexpr int LOGT = __builtin_ctz(T);
     int *ck = ckeys + slot * NMAX;
     int n = 0;
     const unsigned long long below = gl ? ((1ull << gl) - 1ull) : 0ull;
@@ -99,7 +102,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
         const bool valid = key != -1;
         const unsigned long long m = unit_ballot<U>(valid);
         const int pos = n + __popcll(m & below);
-        if (valid && pos < NMAX) ck[pos] = key;
+        if (valid && pos < NMAX) ck[pos] = pack ? ((key << LOGT) | (s * U + gl)) : key;
         n += __popcll(m);
     }
     n = min(n, NMAX);
@@ -118,8 +121,9 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     for (int k = 0; k < K; k++) {
         const int j = k * U + gl;
         if (j < n_out) {
-            const int h = hash_find<T, H>(kt, sk[k]);
-            ccol[off + j] = sk[k];
+            const int key = pack ? (sk[k] >> LOGT) : sk[k];
+            const int h = pack ? (sk[k] & (T - 1)) : hash_find<T, H>(kt, sk[k]);
+            ccol[off + j] = key;
             cval[off + j] = h >= 0 ? static_cast<V>(vt[h]) : V(0);
         }
     }
@@ -825,14 +829,16 @@ void launch_small(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, co
 {
     constexpr int RPB = BS / G;
     const int grid = xcd_grid((nrows + RPB - 1) / RPB);
+    // keys with their slot packed in the low log2(T) bits stay positive
+    const bool pack = static_cast<long long>(B.N) <= (1ll << (31 - __builtin_ctz(T)));
     if (hash == kHashLin)
         k_num_small<G, T, NMAX, BS, kHashLin, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val),
                                                                      B.rpt, B.col, static_cast<const V *>(B.val), rows,
-                                                                     nrows, crpt, ccol, cval);
+                                                                     nrows, crpt, ccol, cval, pack);
     else
         k_num_small<G, T, NMAX, BS, kHashFib, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val),
                                                                      B.rpt, B.col, static_cast<const V *>(B.val), rows,
-                                                                     nrows, crpt, ccol, cval);
+                                                                     nrows, crpt, ccol, cval, pack);
 }
 
 template <int T, int BS, typename V>
