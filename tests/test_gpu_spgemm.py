@@ -387,3 +387,31 @@ def test_block_bins_rank_and_hash_paths(gpu):
         log(f"block bins N={N}")
         run(A, B)
         run(A, B, np.float32)
+
+
+def test_random_products_parity(gpu):
+    """Seeded random products mixing short and long A and B rows, narrow and
+    wide column ranges (up to 2.2M columns), FP64 and FP32, 32- and 64-bit
+    row offsets and all symbolic tunings, each against the oracle."""
+    from opsparse_amd.spgemm import Workspace, default_tuning
+    rng = np.random.default_rng(2024)
+    for case in range(40):
+        K = int(rng.choice([50, 500, 5000, 40000]))
+        N = int(rng.choice([64, 3000, 70000, 140000, 300000, 1100000, 2200000]))
+        M = int(rng.integers(1, 300))
+        la = np.where(rng.random(M) < 0.1, rng.integers(0, min(K, 600) + 1, M), rng.integers(0, min(K, 40) + 1, M))
+        lb = np.where(rng.random(K) < 0.05, rng.integers(0, min(N, 3000) + 1, K), rng.integers(0, min(N, 60) + 1, K))
+        A, B = random_csr(M, K, la, rng), random_csr(K, N, lb, rng)
+        dt = np.float64 if rng.random() < 0.5 else np.float32
+        tdt = torch.float64 if dt == np.float64 else torch.float32
+        wide = bool(rng.random() < 0.3)
+        tun = dict(default_tuning(), sym_bitmap=int(rng.random() < 0.5), short_hash=int(rng.random() < 0.5))
+        log(f"random case {case}: M={M} K={K} N={N} nnz(A)={A.nnz} nnz(B)={B.nnz} {dt.__name__} wide={wide} "
+            f"bitmap={tun['sym_bitmap']} hash={tun['short_hash']}")
+        Ah, Bh = (A, B) if dt == np.float64 else (A.astype(dt), B.astype(dt))
+        ws = Workspace()
+        ws.set_tuning(tun)
+        C = spgemm(DeviceCSR.from_host(Ah, tdt), DeviceCSR.from_host(Bh, tdt), ws=ws, wide=wide)
+        rpt, col, val = C.to_host()
+        check_product(rpt, col, val, A, B, dt)
+        C.free()
