@@ -111,3 +111,28 @@ def test_backward_long_rows_skipped_windows(gpu):
     B = G.HostCSR(N, N, B.rpt, col, B.val)
     check_grad(A, B, torch.float64)
     check_grad(A, B, torch.float32)
+
+
+def test_random_backward_and_plan_parity(gpu):
+    """Seeded random products (short and long rows, narrow and wide spans,
+    FP64 and FP32): the plan's numeric phase against the oracle product and
+    the masked backward against the oracle gradient."""
+    from helpers import check_product
+    rng = np.random.default_rng(77)
+    for case in range(20):
+        K = int(rng.choice([40, 400, 4000, 30000]))
+        N = int(rng.choice([64, 3000, 140000, 300000, 1100000]))
+        M = int(rng.integers(1, 200))
+        la = np.where(rng.random(M) < 0.1, rng.integers(0, min(K, 500) + 1, M), rng.integers(0, min(K, 30) + 1, M))
+        lb = np.where(rng.random(K) < 0.05, rng.integers(0, min(N, 2500) + 1, K), rng.integers(0, min(N, 50) + 1, K))
+        A, B = random_csr(M, K, la, rng), random_csr(K, N, lb, rng)
+        dtype = torch.float64 if rng.random() < 0.5 else torch.float32
+        print(f"random backward case {case}: M={M} K={K} N={N} nnz(A)={A.nnz} nnz(B)={B.nnz} {dtype}", flush=True)
+        npdt = np.float64 if dtype == torch.float64 else np.float32
+        Ah, Bh = (A, B) if npdt == np.float64 else (A.astype(npdt), B.astype(npdt))
+        dA, dB = DeviceCSR.from_host(Ah, dtype), DeviceCSR.from_host(Bh, dtype)
+        plan = Plan(dA, dB)
+        col, val = plan.numeric(dA, dB)
+        torch.cuda.synchronize()
+        check_product(plan.rowptr64().cpu().numpy(), col.cpu().numpy(), val.cpu().numpy(), A, B, npdt)
+        check_grad(A, B, dtype, seed=case)
