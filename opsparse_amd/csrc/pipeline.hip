@@ -114,7 +114,8 @@ struct ops_plan {
     ops_workspace *ws = nullptr;
     PlanCore core;
     DevBuf<long long> crpt64;
-    DevBuf<int> bins, cursor, spill, wcut;
+    DevBuf<int> bins, cursor, spill, wcut, bm_slot;
+    DevBuf<unsigned> bm;  // column bitmaps of long rows kept for the numeric phase
     int annz = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
@@ -212,11 +213,12 @@ constexpr long long kMaxKeptBitmapBytes = 64ll << 30;
 
 // Setup, symbolic and scan phases.  crpt64_dst: where C's 64-bit row
 // offsets go; crpt32_dst (optional): int32 copy, written only if nnz fits.
-// keep_bitmaps: long rows may leave their column bitmaps in the workspace
-// for the numeric phase (pc.ls.bm); plans, whose numeric phase runs later
-// and repeatedly, pass false.
+// kb_bm, kb_slot: where long rows may leave their column bitmaps for the
+// numeric phase (pc.ls.bm): the workspace's buffers for one call, a plan's
+// own for a plan, whose numeric phase runs later and repeatedly.
 void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore &pc, long long *crpt64_dst,
-                    int *crpt32_dst, int *bins_dst, hipStream_t s, ops_timings *t, bool keep_bitmaps)
+                    int *crpt32_dst, int *bins_dst, hipStream_t s, ops_timings *t, DevBuf<unsigned> &kb_bm,
+                    DevBuf<int> &kb_slot)
 {
     const bool sep = t && t->measure_separate;
     const bool ev = t && t->kernel_events;
@@ -284,27 +286,27 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     // ranges are known without a mark walk (one window, or the cut table).
     SymSave save;
     const int long_rows = st.sym_size[kNumBins - 1];
-    if (keep_bitmaps && long_rows > 0 && B.N <= kSymLongWindow &&
+    if (long_rows > 0 && B.N <= kSymLongWindow &&
         (rank_windows(B.N) == 1 || cut_table_wanted(B.M, B.N))) {
         const long long words = (static_cast<long long>(B.N) + 31) / 32;
         size_t free_b = 0, total_b = 0;
         CK(hipMemGetInfo(&free_b, &total_b));
         // The bitmaps this workspace already holds count as free.
-        const long long avail = static_cast<long long>(free_b + ws->bm.cap * sizeof(unsigned));
+        const long long avail = static_cast<long long>(free_b + kb_bm.cap * sizeof(unsigned));
         const long long budget = std::min<long long>(kMaxKeptBitmapBytes, avail / 4);
         long long cap = std::min<long long>(long_rows, budget / (4 * words));
         // OPSAMD_KEPT_BITMAPS=n keeps at most n (0: none), for tests.
         const char *env = std::getenv("OPSAMD_KEPT_BITMAPS");
         if (env && env[0]) cap = std::min(cap, std::atoll(env));
         if (cap > 0) {
-            ws->bm.ensure(static_cast<size_t>(cap * words));
-            ws->bm_slot.ensure(static_cast<size_t>(M) + 1);
+            kb_bm.ensure(static_cast<size_t>(cap * words));
+            kb_slot.ensure(static_cast<size_t>(M) + 1);
             // slot = -1 for every row, count (at [M]) = 0
-            CK(hipMemsetAsync(ws->bm_slot.p, 0xFF, sizeof(int) * static_cast<size_t>(M), s));
-            CK(hipMemsetAsync(ws->bm_slot.p + M, 0, sizeof(int), s));
-            save = SymSave{ws->bm.p, ws->bm_slot.p, ws->bm_slot.p + M, bb.num[kNumBins - 2], static_cast<int>(cap)};
-            pc.ls.bm = ws->bm.p;
-            pc.ls.bm_slot = ws->bm_slot.p;
+            CK(hipMemsetAsync(kb_slot.p, 0xFF, sizeof(int) * static_cast<size_t>(M), s));
+            CK(hipMemsetAsync(kb_slot.p + M, 0, sizeof(int), s));
+            save = SymSave{kb_bm.p, kb_slot.p, kb_slot.p + M, bb.num[kNumBins - 2], static_cast<int>(cap)};
+            pc.ls.bm = kb_bm.p;
+            pc.ls.bm_slot = kb_slot.p;
             pc.ls.bm_words = static_cast<int>(words);
         }
     }
@@ -485,7 +487,7 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
     }
     const double t_alloc_rpt = now_s() - t0;
 
-    symbolic_phase(ws, a, b, pc, crpt64_dst, crpt32_dst, ws->bins.p, s, t, true);
+    symbolic_phase(ws, a, b, pc, crpt64_dst, crpt32_dst, ws->bins.p, s, t, ws->bm, ws->bm_slot);
     if (t) t->setup += t_alloc_rpt;
 
     if (C32 && pc.nnz > static_cast<long long>(INT32_MAX))
@@ -819,7 +821,7 @@ int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_p
         const int M = A->M;
         p->crpt64.ensure(static_cast<size_t>(M) + 1);
         p->bins.ensure(static_cast<size_t>(M) + 1);
-        symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr, false);
+        symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr, p->bm, p->bm_slot);
         numeric_scratch(p->core, A, B, p->cursor, p->spill, p->wcut, s);
         CK(hipStreamSynchronize(s));
         if (nnz_c) *nnz_c = p->core.nnz;
@@ -843,6 +845,8 @@ int ops_plan_destroy(ops_plan *plan)
     plan->cursor.release();
     plan->spill.release();
     plan->wcut.release();
+    plan->bm.release();
+    plan->bm_slot.release();
     delete plan;
     return OPS_OK;
 }

@@ -334,19 +334,33 @@ def test_long_rows_kept_bitmaps(gpu, monkeypatch, N):
     numeric phase (B within 2^20 columns): all rows kept, none kept (every
     window marked again) and only two kept (the rest marked), over one
     bitmap window (whole B rows) and two (window-cut table), with one, two
-    and more value sub-windows per window."""
+    and more value sub-windows per window; through the drop-in call and
+    through a plan (eager and replayed hipGraph)."""
     rng = np.random.default_rng(18)
     K = 1000
     lens = np.r_[rng.integers(350, 450, 3), rng.integers(120, 160, 3), rng.integers(40, 60, 3),
                  rng.integers(1, 3, 2)]
     A = random_csr(len(lens), K, lens, rng)
     B = random_csr(K, N, rng.integers(250, 350, K), rng)
+    ref = oracle.spgemm(A, B)
     for kept in ("", "0", "2"):
         if kept:
             monkeypatch.setenv("OPSAMD_KEPT_BITMAPS", kept)
         log(f"kept bitmaps N={N} limit={kept or 'none'}")
         run(A, B)
         run(A, B, np.float32)
+        # a plan keeps its own bitmaps for its eager and replayed numeric phases
+        dA, dB = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+        p = Plan(dA, dB)
+        rpt = p.rowptr64().cpu().numpy()
+        col, val = p.numeric(dA, dB)
+        oracle.compare(rpt, col.cpu().numpy(), val.cpu().numpy(), ref, TOL[np.float64])
+        for _ in range(2):
+            val.fill_(7.0)
+            p.numeric(dA, dB, col, val, graph=True)
+            torch.cuda.synchronize()
+            oracle.compare(rpt, col.cpu().numpy(), val.cpu().numpy(), ref, TOL[np.float64])
+        del p
 
 
 def test_long_rows_multi_window_cut_table_and_gallop(gpu, monkeypatch):
