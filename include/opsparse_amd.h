@@ -91,6 +91,9 @@ typedef struct ops_timings {
     int64_t num_nnz[OPS_NUM_BINS];
     int64_t num_arows_nnz[OPS_NUM_BINS]; /* nnz of A over the rows of the bin */
     double num_ms[OPS_NUM_BINS];
+    /* out: long rows whose symbolic column bitmap was kept for the numeric
+     * phase (see ops_kept_bitmap_limit) */
+    int64_t kept_rows;
 } ops_timings;
 
 /* Persistent per-device workspace: stream pool, events, growable device
@@ -102,6 +105,23 @@ int ops_workspace_create(ops_workspace **ws);
 int ops_workspace_destroy(ops_workspace *ws);
 /* Last error message recorded on this workspace (never NULL). */
 const char *ops_workspace_last_error(const ops_workspace *ws);
+/* Frees the workspace's growable device buffers (bin lists, scan sums,
+ * cursors, window-cut table, kept column bitmaps); streams and events stay,
+ * and the next call allocates what it needs again.  The counterpart of the
+ * per-call SpgemmMeta::release (meta.cu:50-65) for callers that want the
+ * memory back between products. */
+int ops_workspace_trim(ops_workspace *ws);
+
+/* Long-row column bitmaps.  For rows above the numeric bin-6 bound, the
+ * symbolic phase can keep each row's column bitmap (ceil(B.N / 32) words,
+ * 128 KB at 2^20 columns) so the numeric phase skips a second walk over the
+ * row's products.  A workspace keeps them from one call to the next and a
+ * plan until it is destroyed.  Every workspace and plan of the process on
+ * one device shares one limit: `bytes` (< 0, the default: min(64 GiB, a
+ * quarter of the device's memory); 0: keep none).  Each holder also takes
+ * at most a quarter of the free memory, and when the allocation fails the
+ * rows are marked again instead (never an error). */
+int ops_kept_bitmap_limit(int64_t bytes);
 
 /*
  * C = A * B.  Replaces opsparse::opsparse_spgemm / opsparse_spgemm_reg
@@ -176,12 +196,17 @@ int ops_compute_flop_device(ops_workspace *ws, const ops_csr *A, const ops_csr *
 
 /* A plan holds C's row offsets and the numeric row bins for fixed patterns
  * of A and B (no counterpart in the reference).  ops_plan_create runs the
- * setup, symbolic and scan phases and reports nnz(C). */
+ * setup, symbolic and scan phases and reports nnz(C).  A plan holds device
+ * memory until it is destroyed: C's row offsets, the bin lists, the
+ * long-row scratch and the kept column bitmaps of its long rows (within
+ * ops_kept_bitmap_limit). */
 typedef struct ops_plan ops_plan;
 
 int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_plan **plan, int64_t *nnz_c,
                     void *stream);
 int ops_plan_destroy(ops_plan *plan);
+/* Long rows whose column bitmap the plan keeps. */
+int ops_plan_kept_rows(const ops_plan *plan, int64_t *rows);
 /* Copies C's row offsets (M+1) into caller device memory, int32 or int64. */
 int ops_plan_rowptr32(const ops_plan *plan, int32_t *rpt, void *stream);
 int ops_plan_rowptr64(const ops_plan *plan, int64_t *rpt, void *stream);
@@ -235,7 +260,10 @@ int ops_batch_pack(int32_t count, const ops_csr *instances, ops_dtype dt, ops_cs
 
 /* Given dC on pattern(C) (ccol sorted per row, row offsets crpt64),
  * dA = (dC * B^T) sampled on pattern(A), dB = (A^T * dC) sampled on
- * pattern(B).  da/db are overwritten (nnz(A) / nnz(B) values). */
+ * pattern(B).  da and db are overwritten and are indexed from the first
+ * entry of each descriptor: da[e - A->rpt[0]] is the gradient of A entry e
+ * (A->rpt[M] - A->rpt[0] values), db[e - B->rpt[0]] that of B entry e, so a
+ * row-slice descriptor's gradients fill a buffer of the slice's nnz. */
 int ops_spgemm_backward(ops_workspace *ws, ops_dtype dt, const ops_csr *A, const ops_csr *B, const int64_t *crpt64,
                         const int32_t *ccol, const void *dcval, void *da, void *db, void *stream);
 

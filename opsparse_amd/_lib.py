@@ -53,7 +53,7 @@ class OpsTimings(C.Structure):
                 ("sym_ms", C.c_double * NUM_BINS),
                 ("num_rows", C.c_int64 * NUM_BINS), ("num_flop", C.c_int64 * NUM_BINS),
                 ("num_nnz", C.c_int64 * NUM_BINS), ("num_arows_nnz", C.c_int64 * NUM_BINS),
-                ("num_ms", C.c_double * NUM_BINS)]
+                ("num_ms", C.c_double * NUM_BINS), ("kept_rows", C.c_int64)]
 
 
 class OpsTuning(C.Structure):
@@ -96,6 +96,8 @@ SIGNATURES = {
     "ops_workspace_create": (C.c_int, [_P(_vp)]),
     "ops_workspace_destroy": (C.c_int, [_vp]),
     "ops_workspace_last_error": (C.c_char_p, [_vp]),
+    "ops_workspace_trim": (C.c_int, [_vp]),
+    "ops_kept_bitmap_limit": (C.c_int, [C.c_int64]),
     "ops_spgemm_f64": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _P(OpsCsr), _vp, _P(OpsTimings)]),
     "ops_spgemm_f32": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _P(OpsCsr), _vp, _P(OpsTimings)]),
     "ops_spgemm64_f64": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _P(OpsCsr64), _vp, _P(OpsTimings)]),
@@ -113,6 +115,7 @@ SIGNATURES = {
     "ops_compute_flop_device": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _vp, _P(C.c_int64), _vp]),
     "ops_plan_create": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _P(_vp), _P(C.c_int64), _vp]),
     "ops_plan_destroy": (C.c_int, [_vp]),
+    "ops_plan_kept_rows": (C.c_int, [_vp, _P(C.c_int64)]),
     "ops_plan_rowptr32": (C.c_int, [_vp, _vp, _vp]),
     "ops_plan_rowptr64": (C.c_int, [_vp, _vp, _vp]),
     "ops_plan_numeric": (C.c_int, [_vp, C.c_int, _P(OpsCsr), _P(OpsCsr), _vp, _vp, _vp]),

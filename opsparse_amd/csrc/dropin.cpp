@@ -207,10 +207,7 @@ void CSR::construct(const std::string &mtx_file)
     std::vector<E> ents;
     ents.reserve(static_cast<size_t>(nz) * (banner.symmetry == "general" ? 1 : 2));
     const bool pattern = banner.type == "pattern";
-    const bool cplx = banner.type == "complex";
-    if (!(pattern || cplx || banner.type == "real" || banner.type == "integer"))
-        throw std::runtime_error("invalid MatrixMarket data type");
-    if (banner.symmetry == "hermitian") throw std::runtime_error("MatrixMarket I/O does not support hermitian matrices");
+    const bool cplx = banner.type == "complex";  // the imaginary part is read and dropped
     long long read = 0;
     while (read < nz) {
         long long i, j;
@@ -246,39 +243,6 @@ void CSR::construct(const std::string &mtx_file)
     }
     for (mint i = 0; i < M; i++) rpt[i + 1] += rpt[i];
 }
-
-// ---------------- MatrixMarket banner ----------------
-
-namespace io {
-
-void tokenize(std::vector<std::string> &tokens, const std::string &line)
-{
-    tokens.clear();
-    std::istringstream ss(line);
-    std::string t;
-    while (ss >> t) tokens.push_back(t);
-}
-
-void read_mm_banner(std::istream &in, MatrixMarketBanner &banner)
-{
-    std::string line;
-    if (!std::getline(in, line)) throw std::runtime_error("MatrixMarket: empty file");
-    std::vector<std::string> tok;
-    tokenize(tok, line);
-    if (tok.size() != 5 || tok[0] != "%%MatrixMarket") throw std::runtime_error("MatrixMarket banner not found");
-    for (auto &s : tok) std::transform(s.begin(), s.end(), s.begin(), [](unsigned char c) { return std::tolower(c); });
-    banner.storage = tok[1];
-    banner.format = tok[2];
-    banner.type = tok[3];
-    banner.symmetry = tok[4];
-    if (banner.storage != "matrix") throw std::runtime_error("MatrixMarket: only matrix storage is supported");
-    if (banner.format != "coordinate") throw std::runtime_error("MatrixMarket: only coordinate format is supported");
-    if (banner.symmetry != "general" && banner.symmetry != "symmetric" && banner.symmetry != "skew-symmetric" &&
-        banner.symmetry != "hermitian")
-        throw std::runtime_error("MatrixMarket: invalid symmetry");
-}
-
-}  // namespace io
 
 // ---------------- SpgemmMeta ----------------
 

@@ -15,7 +15,10 @@
 //    C hold no products and are skipped by reading the next column of the C
 //    row.
 // dA is reduced per A-nonzero in LDS, then added to da once per chunk; dB is
-// scattered with global atomic adds.
+// scattered with global atomic adds.  da and db hold one value per entry of
+// A's and B's descriptors counted from their first entry: entry e of A (an
+// index into A.col) is da[e - a0], a0 = A.rpt[0], and likewise for B, so a
+// row slice's gradients fill a buffer of the slice's own nnz.
 #include <algorithm>
 
 #include "device.hpp"
@@ -38,6 +41,7 @@ template <int U, typename V>
 struct DaReduce {
     Acc *sda;
     V *__restrict__ da;
+    int a0;  // first entry of A's descriptor (da[e - a0] holds entry e)
     __device__ __forceinline__ void add(int q, Acc v) { atomicAdd(sda + q, v); }
     __device__ __forceinline__ void flush(int e, bool valid, bool shared)
     {
@@ -48,9 +52,9 @@ struct DaReduce {
         wave_lds_sync();
         if (valid && v != Acc(0)) {
             if (shared)
-                global_add(da + e, static_cast<V>(v));
+                global_add(da + (e - a0), static_cast<V>(v));
             else
-                da[e] += static_cast<V>(v);
+                da[e - a0] += static_cast<V>(v);
         }
     }
 };
@@ -63,6 +67,7 @@ struct BwdHash {
     const int *kt;
     const V *vt;
     V *__restrict__ db;
+    int b0;  // first entry of B's descriptor
     DaReduce<U, V> red;
     struct Item {
         int k;
@@ -74,7 +79,7 @@ struct BwdHash {
     {
         const int h = hash_find<T, H>(kt, it.c);
         const V g = h >= 0 ? vt[h] : V(0);
-        global_add(db + it.k, a * g);
+        global_add(db + (it.k - b0), a * g);
         red.add(q, static_cast<Acc>(g) * static_cast<Acc>(it.b));
     }
     __device__ __forceinline__ void chunk_end(int e, bool valid, bool shared) { red.flush(e, valid, shared); }
@@ -88,7 +93,8 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
                                                  const int *__restrict__ bcol, const V *__restrict__ bval,
                                                  const int *__restrict__ rows, int nrows,
                                                  const long long *__restrict__ crpt, const int *__restrict__ ccol,
-                                                 const V *__restrict__ dc, V *__restrict__ da, V *__restrict__ db)
+                                                 const V *__restrict__ dc, V *__restrict__ da, V *__restrict__ db,
+                                                 int a0, int b0)
 {
     constexpr int U = G < kWave ? G : kWave;
     constexpr int RPB = BS / G;
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(BS) void k_bwd_hash(const int *__restrict__ arpt, c
     }
     __syncthreads();
     if (ridx >= nrows) return;
-    BwdHash<T, H, U, V> op{bcol, bval, kt, vt, db, DaReduce<U, V>{sda + (tid / U) * U, da}};
+    BwdHash<T, H, U, V> op{bcol, bval, kt, vt, db, b0, DaReduce<U, V>{sda + (tid / U) * U, da, a0}};
     const FullRow range{brpt};
     for_each_product<U, true, V, (G > 256)>(arpt[r], arpt[r + 1], unit, NU, acol, aval, range, op);
 }
@@ -173,6 +179,7 @@ struct BwdRank {
     int r0;
     int base;
     V *__restrict__ db;
+    int b0;  // first entry of B's descriptor
     Acc *sda;
     struct Item {
         int k;
@@ -185,7 +192,7 @@ struct BwdRank {
     {
         const int c = it.c - w0;
         const V g = gv[static_cast<int>(e.y) + __popc(e.x & ((1u << (c & 31)) - 1u)) - r0];
-        global_add(db + it.k, a * g);
+        global_add(db + (it.k - b0), a * g);
         atomicAdd(sda + (q - base), static_cast<Acc>(g) * static_cast<Acc>(it.b));
     }
 };
@@ -206,7 +213,7 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
                                                       const long long *__restrict__ crpt,
                                                       const int *__restrict__ ccol, const V *__restrict__ dc,
                                                       V *__restrict__ da, V *__restrict__ db, int *__restrict__ cursor,
-                                                      int span)
+                                                      int span, int a0, int b0)
 {
     using R = BwdRankWin<V>;
     constexpr int NU = kLongBS / kWave;
@@ -276,7 +283,7 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
                 int nxt = INT_MAX;
                 const CursorRange range{brpt, bcol, cursor + (parity ? span : 0), cursor + (parity ? 0 : span), cb,
                                         first, cb >= ncols, &nxt};
-                BwdRank<V> op{bcol, bval, br, gv, w0, r0, 0, db, sda};
+                BwdRank<V> op{bcol, bval, br, gv, w0, r0, 0, db, b0, sda};
                 for (int c = a_beg; c < a_end; c += kLongBS) {
                     int total;
                     const int n = block_stage<kLongBS, V>(c, a_end, acol, aval, range, s_bst, total);
@@ -286,7 +293,7 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
                     if (e < a_end) {
                         const Acc v = sda[tid];
                         if (v != Acc(0)) {
-                            da[e] += static_cast<V>(v);
+                            da[e - a0] += static_cast<V>(v);
                             sda[tid] = Acc(0);
                         }
                     }
@@ -311,13 +318,13 @@ constexpr size_t bwd_lds()
 
 template <int G, int T, int BS, typename V>
 void launch_hash(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt, const int *ccol,
-                 const V *dc, V *da, V *db, hipStream_t s)
+                 const V *dc, V *da, V *db, int a0, int b0, hipStream_t s)
 {
     constexpr int RPB = BS / G;
     const int grid = (nrows + RPB - 1) / RPB;
     k_bwd_hash<G, T, BS, V><<<grid, BS, bwd_lds<G, T, BS, V>(), s>>>(
         A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows, crpt,
-        ccol, dc, da, db);
+        ccol, dc, da, db, a0, b0);
 }
 
 template <int G, int T, int BS, typename V>
@@ -351,29 +358,32 @@ void backward_init_attributes()
 
 template <typename V>
 void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                         const int *ccol, const V *dc, V *da, V *db, int *cursor, int span, hipStream_t s)
+                         const int *ccol, const V *dc, V *da, V *db, int a0, int b0, int *cursor, int span,
+                         hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
-        case 0: launch_hash<16, 32, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
-        case 1: launch_hash<32, 64, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
-        case 2: launch_hash<64, 128, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
-        case 3: launch_hash<64, 256, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
-        case 4: launch_hash<64, 512, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
-        case 5: launch_hash<256, 2048, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
-        case 6: launch_hash<1024, 8192, 1024, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, s); break;
+        case 0: launch_hash<16, 32, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, a0, b0, s); break;
+        case 1: launch_hash<32, 64, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, a0, b0, s); break;
+        case 2: launch_hash<64, 128, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, a0, b0, s); break;
+        case 3: launch_hash<64, 256, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, a0, b0, s); break;
+        case 4: launch_hash<64, 512, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, a0, b0, s); break;
+        case 5: launch_hash<256, 2048, 256, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, a0, b0, s); break;
+        case 6: launch_hash<1024, 8192, 1024, V>(A, B, rows, nrows, crpt, ccol, dc, da, db, a0, b0, s); break;
         default:
             k_bwd_rank<V><<<nrows, kLongBS, BwdRankWin<V>::lds, s>>>(
                 A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
-                B.N, crpt, ccol, dc, da, db, cursor, span);
+                B.N, crpt, ccol, dc, da, db, cursor, span, a0, b0);
             break;
     }
 }
 
 template void launch_backward_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *,
-                                          const int *, const double *, double *, double *, int *, int, hipStream_t);
+                                          const int *, const double *, double *, double *, int, int, int *, int,
+                                          hipStream_t);
 template void launch_backward_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *,
-                                         const int *, const float *, float *, float *, int *, int, hipStream_t);
+                                         const int *, const float *, float *, float *, int, int, int *, int,
+                                         hipStream_t);
 
 void launch_zero_values(void *p, long long n, bool f32, hipStream_t s)
 {

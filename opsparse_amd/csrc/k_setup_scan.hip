@@ -370,6 +370,13 @@ __global__ void k_narrow(const long long *__restrict__ src, int n, int *__restri
     if (i < n) dst[i] = static_cast<int>(src[i]);
 }
 
+__global__ void k_fill_int(int *__restrict__ p, int v, long long n)
+{
+    for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
 __global__ void k_rownnz(const long long *__restrict__ crpt, int M, int *__restrict__ row_nnz)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,6 +389,13 @@ void launch_rownnz_from_rpt(const long long *crpt, int M, int *row_nnz, hipStrea
 {
     if (M <= 0) return;
     k_rownnz<<<(M + 255) / 256, 256, 0, s>>>(crpt, M, row_nnz);
+}
+
+void launch_fill_int(int *p, int v, long long n, hipStream_t s)
+{
+    if (n <= 0) return;
+    const int grid = static_cast<int>(std::min<long long>((n + 255) / 256, 16384));
+    k_fill_int<<<grid, 256, 0, s>>>(p, v, n);
 }
 
 void launch_zero_stats(DevStats *st, hipStream_t s) { k_zero_stats<<<1, 256, 0, s>>>(st); }

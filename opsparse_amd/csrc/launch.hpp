@@ -72,6 +72,8 @@ struct DevStats {
     unsigned long long num_flop[kNumBins];
     unsigned long long num_nnz[kNumBins];
     unsigned long long num_annz[kNumBins];
+    int kept_rows;  // long rows that asked for a kept column bitmap (SymSave::count)
+    int pad_;
 };
 
 struct CsrIn {
@@ -96,6 +98,8 @@ int scan_blocks(int M);
 void launch_scan(const int *row_nnz, const int *row_flop, const int *arpt, int M, long long *block_sums,
                  long long *crpt64, int *crpt32, DevStats *st, const BinBounds &bb, hipStream_t s);
 void launch_narrow_rpt(const long long *crpt64, int M, int *crpt32, hipStream_t s);
+// p[0..n) = v with a kernel (no memset node when captured in a graph).
+void launch_fill_int(int *p, int v, long long n, hipStream_t s);
 
 // --- symbolic (k_symbolic.hip) ---
 // Columns per bitmap window of the long-row symbolic kernel (128 KB of LDS).
@@ -160,9 +164,12 @@ void numeric_init_attributes();
 void symbolic_init_attributes();
 
 // --- backward (k_backward.hip) ---
+// da[e - a0] receives the gradient of A entry e (a0 = A.rpt[0]), db[e - b0]
+// that of B entry e (b0 = B.rpt[0]).
 template <typename V>
 void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                         const int *ccol, const V *dc, V *da, V *db, int *cursor, int span, hipStream_t s);
+                         const int *ccol, const V *dc, V *da, V *db, int a0, int b0, int *cursor, int span,
+                         hipStream_t s);
 void backward_init_attributes();
 void launch_zero_values(void *p, long long n, bool f32, hipStream_t s);
 // row_nnz[i] = crpt[i+1] - crpt[i] (numeric binning key for the backward).

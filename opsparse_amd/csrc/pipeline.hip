@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -76,6 +77,49 @@ struct DevBuf {
 
 bool g_attrs_done = false;
 
+// Bytes of long-row column bitmaps held per device by every workspace and
+// plan of the process together, against one limit (ops_kept_bitmap_limit).
+constexpr int kMaxDev = 64;
+constexpr long long kMaxKeptBitmapBytes = 64ll << 30;
+std::mutex g_kb_mu;
+long long g_kb_held[kMaxDev] = {};
+long long g_kb_limit = -1;  // < 0: min(kMaxKeptBitmapBytes, a quarter of the device's memory)
+
+// Column bitmaps one holder (a workspace or a plan) keeps between the
+// symbolic and numeric phases; its bytes count against the device's limit.
+struct KeptBitmaps {
+    DevBuf<unsigned> bm;
+    DevBuf<int> slot;
+    int dev = -1;
+    long long accounted = 0;
+    // Bytes this holder may hold: the device limit minus what the others
+    // hold, and at most a quarter of free memory (counting its own as free).
+    long long budget(size_t free_b, size_t total_b)
+    {
+        std::lock_guard<std::mutex> g(g_kb_mu);
+        const long long lim = g_kb_limit >= 0 ? g_kb_limit
+                                              : std::min<long long>(kMaxKeptBitmapBytes,
+                                                                    static_cast<long long>(total_b) / 4);
+        const int d = dev >= 0 ? dev : 0;
+        const long long others = g_kb_held[d] - accounted;
+        const long long by_free = (static_cast<long long>(free_b) + accounted) / 4;
+        return std::max<long long>(0, std::min(lim - others, by_free));
+    }
+    void account()
+    {
+        std::lock_guard<std::mutex> g(g_kb_mu);
+        const long long now = static_cast<long long>(bm.cap * sizeof(unsigned));
+        if (dev >= 0 && dev < kMaxDev) g_kb_held[dev] += now - accounted;
+        accounted = now;
+    }
+    void release()
+    {
+        bm.release();
+        slot.release();
+        account();
+    }
+};
+
 }  // namespace
 
 struct ops_workspace {
@@ -87,8 +131,8 @@ struct ops_workspace {
     hipEvent_t ev_k0[kNumBins] = {}, ev_k1[kNumBins] = {};
     DevStats *d_stats = nullptr;
     DevStats *h_stats = nullptr;  // pinned
-    DevBuf<int> row_flop, row_nnz, bins, cursor, spill, wcut, crpt32_tmp, bm_slot;
-    DevBuf<unsigned> bm;  // column bitmaps kept by the symbolic phase
+    DevBuf<int> row_flop, row_nnz, bins, cursor, spill, wcut, crpt32_tmp;
+    KeptBitmaps kb;  // column bitmaps kept by the symbolic phase for one call
     DevBuf<long long> block_sums, crpt64;
     BinBounds bounds = default_bounds();
     int short_hash = kHashLin;
@@ -107,6 +151,7 @@ struct PlanCore {
     int num_size[kNumBins] = {};
     int num_off[kNumBins] = {};
     int hash = kHashLin;  // hash family of the short-row bins
+    int kept_rows = 0;    // long rows whose symbolic column bitmap is kept (ls.bm)
     DevStats stats{};
 };
 
@@ -114,8 +159,8 @@ struct ops_plan {
     ops_workspace *ws = nullptr;
     PlanCore core;
     DevBuf<long long> crpt64;
-    DevBuf<int> bins, cursor, spill, wcut, bm_slot;
-    DevBuf<unsigned> bm;  // column bitmaps of long rows kept for the numeric phase
+    DevBuf<int> bins, cursor, spill, wcut;
+    KeptBitmaps kb;  // column bitmaps of long rows kept for every numeric phase of the plan
     int annz = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
@@ -207,18 +252,13 @@ bool cut_table_wanted(int bm, int bn)
     return nwin > 1 && bytes <= kMaxCutBytes && !off;
 }
 
-// Most bytes of column bitmaps the symbolic phase keeps for the numeric
-// phase (further long rows are marked again by the numeric kernel).
-constexpr long long kMaxKeptBitmapBytes = 64ll << 30;
-
 // Setup, symbolic and scan phases.  crpt64_dst: where C's 64-bit row
 // offsets go; crpt32_dst (optional): int32 copy, written only if nnz fits.
-// kb_bm, kb_slot: where long rows may leave their column bitmaps for the
-// numeric phase (pc.ls.bm): the workspace's buffers for one call, a plan's
-// own for a plan, whose numeric phase runs later and repeatedly.
+// kb: where long rows may leave their column bitmaps for the numeric phase
+// (pc.ls.bm): the workspace's buffers for one call, a plan's own for a
+// plan, whose numeric phase runs later and repeatedly.
 void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore &pc, long long *crpt64_dst,
-                    int *crpt32_dst, int *bins_dst, hipStream_t s, ops_timings *t, DevBuf<unsigned> &kb_bm,
-                    DevBuf<int> &kb_slot)
+                    int *crpt32_dst, int *bins_dst, hipStream_t s, ops_timings *t, KeptBitmaps &kb)
 {
     const bool sep = t && t->measure_separate;
     const bool ev = t && t->kernel_events;
@@ -291,22 +331,31 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
         const long long words = (static_cast<long long>(B.N) + 31) / 32;
         size_t free_b = 0, total_b = 0;
         CK(hipMemGetInfo(&free_b, &total_b));
-        // The bitmaps this workspace already holds count as free.
-        const long long avail = static_cast<long long>(free_b + kb_bm.cap * sizeof(unsigned));
-        const long long budget = std::min<long long>(kMaxKeptBitmapBytes, avail / 4);
-        long long cap = std::min<long long>(long_rows, budget / (4 * words));
+        if (kb.dev < 0) kb.dev = ws->device;
+        long long cap = std::min<long long>(long_rows, kb.budget(free_b, total_b) / (4 * words));
         // OPSAMD_KEPT_BITMAPS=n keeps at most n (0: none), for tests.
         const char *env = std::getenv("OPSAMD_KEPT_BITMAPS");
         if (env && env[0]) cap = std::min(cap, std::atoll(env));
         if (cap > 0) {
-            kb_bm.ensure(static_cast<size_t>(cap * words));
-            kb_slot.ensure(static_cast<size_t>(M) + 1);
-            // slot = -1 for every row, count (at [M]) = 0
-            CK(hipMemsetAsync(kb_slot.p, 0xFF, sizeof(int) * static_cast<size_t>(M), s));
-            CK(hipMemsetAsync(kb_slot.p + M, 0, sizeof(int), s));
-            save = SymSave{kb_bm.p, kb_slot.p, kb_slot.p + M, bb.num[kNumBins - 2], static_cast<int>(cap)};
-            pc.ls.bm = kb_bm.p;
-            pc.ls.bm_slot = kb_slot.p;
+            // Keeping bitmaps is an optimisation: when their memory cannot
+            // be had, the numeric phase marks the rows again.
+            try {
+                kb.bm.ensure(static_cast<size_t>(cap * words));
+                kb.slot.ensure(static_cast<size_t>(M));
+            } catch (const HipFail &e) {
+                if (e.status != OPS_OOM) throw;
+                kb.release();
+                cap = 0;
+            }
+            kb.account();
+        }
+        if (cap > 0) {
+            // slot = -1 for every row; the kept-row counter (in d_stats) is zero
+            launch_fill_int(kb.slot.p, -1, M, s);
+            CK(hipGetLastError());
+            save = SymSave{kb.bm.p, kb.slot.p, &ws->d_stats->kept_rows, bb.num[kNumBins - 2], static_cast<int>(cap)};
+            pc.ls.bm = kb.bm.p;
+            pc.ls.bm_slot = kb.slot.p;
             pc.ls.bm_words = static_cast<int>(words);
         }
     }
@@ -331,9 +380,11 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     st = *ws->h_stats;
     pc.stats = st;
     pc.nnz = static_cast<long long>(st.total_nnz);
+    pc.kept_rows = save.bm ? std::min(st.kept_rows, save.cap) : 0;
     if (t) {
         t->prefix = now_s() - t0;
         t->nnz_c = pc.nnz;
+        t->kept_rows = pc.kept_rows;
         t->max_row_nnz = st.max_nnz;
         for (int b = 0; b < kNumBins; b++) {
             t->num_rows[b] = st.num_size[b];
@@ -408,6 +459,21 @@ long long a_index_span(const ops_csr *A, hipStream_t s)
     CK(hipMemcpyAsync(&last, A->rpt + A->M, sizeof(int), hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
     return last;
+}
+
+// Entries [rpt[0], rpt[M]) of a descriptor, read from the device.
+struct EntryRange {
+    long long begin = 0, end = 0;
+};
+EntryRange entry_range(const ops_csr *X, hipStream_t s)
+{
+    if (X->M == 0) return {};
+    int v[2] = {0, 0};
+    CK(hipMemcpyAsync(&v[0], X->rpt, sizeof(int), hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(&v[1], X->rpt + X->M, sizeof(int), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    if (v[1] < v[0]) throw HipFail(OPS_INVALID, "descriptor row offsets decrease");
+    return EntryRange{v[0], v[1]};
 }
 
 // Scratch of the numeric phase: the spill list of bin kSpillBin and, for the
@@ -487,7 +553,7 @@ int spgemm_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr *
     }
     const double t_alloc_rpt = now_s() - t0;
 
-    symbolic_phase(ws, a, b, pc, crpt64_dst, crpt32_dst, ws->bins.p, s, t, ws->bm, ws->bm_slot);
+    symbolic_phase(ws, a, b, pc, crpt64_dst, crpt32_dst, ws->bins.p, s, t, ws->kb);
     if (t) t->setup += t_alloc_rpt;
 
     if (C32 && pc.nnz > static_cast<long long>(INT32_MAX))
@@ -537,9 +603,12 @@ void backward_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const 
     ws->bins.ensure(static_cast<size_t>(M) + 1);
     ws->block_sums.ensure(static_cast<size_t>(scan_blocks(M)) + 2);
     const long long *crpt = reinterpret_cast<const long long *>(crpt64);
-    // dA entries of rows without products and all of dB start at zero.
-    launch_zero_values(da, a_index_span(A, s), sizeof(V) == 4, s);
-    launch_zero_values(db, a_index_span(B, s), sizeof(V) == 4, s);
+    // da and db are indexed from the first entry of A's and B's descriptors
+    // (row slices need not start at entry 0).  dA entries of rows without
+    // products and all of dB start at zero.
+    const EntryRange ea = entry_range(A, s), eb = entry_range(B, s);
+    launch_zero_values(da, ea.end - ea.begin, sizeof(V) == 4, s);
+    launch_zero_values(db, eb.end - eb.begin, sizeof(V) == 4, s);
     launch_zero_stats(ws->d_stats, s);
     launch_rownnz_from_rpt(crpt, M, ws->row_nnz.p, s);
     CK(hipGetLastError());
@@ -573,13 +642,14 @@ void backward_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const 
     int *cursor = nullptr;
     int span = 0;
     if (st.num_size[kNumBins - 1] > 0) {
-        span = static_cast<int>(a_index_span(A, s));
+        span = static_cast<int>(ea.end);
         ws->cursor.ensure(2 * static_cast<size_t>(span) + 2);
         cursor = ws->cursor.p;
     }
+    const int a0 = static_cast<int>(ea.begin), b0 = static_cast<int>(eb.begin);
     dispatch_bins(ws, s, st.num_size, false, [&](int bin, hipStream_t ps) {
         launch_backward_bin<V>(bin, a, b, rows ? rows + off[bin] : nullptr, st.num_size[bin], crpt, ccol, dc, da, db,
-                               cursor, span, ps);
+                               a0, b0, cursor, span, ps);
     });
 }
 
@@ -636,8 +706,7 @@ int ops_workspace_destroy(ops_workspace *ws)
     ws->cursor.release();
     ws->spill.release();
     ws->wcut.release();
-    ws->bm.release();
-    ws->bm_slot.release();
+    ws->kb.release();
     ws->crpt32_tmp.release();
     ws->block_sums.release();
     ws->crpt64.release();
@@ -646,6 +715,39 @@ int ops_workspace_destroy(ops_workspace *ws)
 }
 
 const char *ops_workspace_last_error(const ops_workspace *ws) { return ws ? ws->err.c_str() : "null workspace"; }
+
+int ops_workspace_trim(ops_workspace *ws)
+{
+    if (!ws) return OPS_INVALID;
+    return guarded(ws, [&]() {
+        CK(hipDeviceSynchronize());
+        ws->row_flop.release();
+        ws->row_nnz.release();
+        ws->bins.release();
+        ws->cursor.release();
+        ws->spill.release();
+        ws->wcut.release();
+        ws->kb.release();
+        ws->crpt32_tmp.release();
+        ws->block_sums.release();
+        ws->crpt64.release();
+        return OPS_OK;
+    });
+}
+
+int ops_kept_bitmap_limit(int64_t bytes)
+{
+    std::lock_guard<std::mutex> g(g_kb_mu);
+    g_kb_limit = bytes < 0 ? -1 : bytes;
+    return OPS_OK;
+}
+
+int ops_plan_kept_rows(const ops_plan *plan, int64_t *rows)
+{
+    if (!plan || !rows) return OPS_INVALID;
+    *rows = plan->core.kept_rows;
+    return OPS_OK;
+}
 
 int ops_tuning_default(ops_tuning *t)
 {
@@ -821,7 +923,7 @@ int ops_plan_create(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_p
         const int M = A->M;
         p->crpt64.ensure(static_cast<size_t>(M) + 1);
         p->bins.ensure(static_cast<size_t>(M) + 1);
-        symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr, p->bm, p->bm_slot);
+        symbolic_phase(ws, as_in(A), as_in(B), p->core, p->crpt64.p, nullptr, p->bins.p, s, nullptr, p->kb);
         numeric_scratch(p->core, A, B, p->cursor, p->spill, p->wcut, s);
         CK(hipStreamSynchronize(s));
         if (nnz_c) *nnz_c = p->core.nnz;
@@ -845,8 +947,7 @@ int ops_plan_destroy(ops_plan *plan)
     plan->cursor.release();
     plan->spill.release();
     plan->wcut.release();
-    plan->bm.release();
-    plan->bm_slot.release();
+    plan->kb.release();
     delete plan;
     return OPS_OK;
 }

@@ -55,16 +55,29 @@ def dtype_code(dtype: torch.dtype) -> int:
 
 @dataclass
 class DeviceCSR:
-    """CSR on the GPU as torch tensors: rpt int32 (M+1), col int32, val f32/f64."""
+    """CSR on the GPU as torch tensors: rpt int32 (M+1), col int32, val f32/f64.
+
+    rpt need not start at 0 (a row slice shares col/val with its parent);
+    nnz is then rpt[M] - rpt[0], given as `entries` (row_slice sets it), and
+    col.numel() otherwise."""
     M: int
     N: int
     rpt: torch.Tensor
     col: torch.Tensor
     val: torch.Tensor
+    entries: int | None = None
 
     @property
     def nnz(self) -> int:
-        return int(self.col.numel())
+        return int(self.col.numel()) if self.entries is None else int(self.entries)
+
+    def row_slice(self, r0: int, r1: int) -> "DeviceCSR":
+        """Rows [r0, r1) as a descriptor over the same col/val storage."""
+        if not 0 <= r0 <= r1 <= self.M:
+            raise ValueError(f"row range [{r0}, {r1}) outside [0, {self.M}]")
+        ends = self.rpt[[r0, r1]].cpu()
+        return DeviceCSR(r1 - r0, self.N, self.rpt[r0:r1 + 1], self.col, self.val,
+                         int(ends[1]) - int(ends[0]))
 
     @staticmethod
     def from_host(h: HostCSR, dtype=torch.float64, device="cuda") -> "DeviceCSR":
@@ -79,7 +92,7 @@ class DeviceCSR:
         return d
 
     def with_values(self, val: torch.Tensor) -> "DeviceCSR":
-        return DeviceCSR(self.M, self.N, self.rpt, self.col, val)
+        return DeviceCSR(self.M, self.N, self.rpt, self.col, val, self.entries)
 
 
 class Workspace:
@@ -100,6 +113,10 @@ class Workspace:
         t = OpsTuning()
         check(lib().ops_workspace_get_tuning(self.ptr, C.byref(t)))
         return t.as_dict()
+
+    def trim(self) -> None:
+        """Frees the workspace's device buffers (ops_workspace_trim)."""
+        check(lib().ops_workspace_trim(self.ptr), self.ptr)
 
     def __del__(self):
         try:
@@ -125,6 +142,11 @@ def set_mem_cache_limit(nbytes: int) -> None:
 def mem_cache_release() -> None:
     """Frees every cached output block."""
     check(lib().ops_mem_cache_release())
+
+
+def set_kept_bitmap_limit(nbytes: int) -> None:
+    """Caps the long-row column bitmaps kept per device (< 0: default, 0: none)."""
+    check(lib().ops_kept_bitmap_limit(int(nbytes)))
 
 
 def default_tuning() -> dict:
@@ -300,6 +322,13 @@ class Plan:
         self.nnz = int(nnz.value)
         self._rpt64 = None
 
+    @property
+    def kept_rows(self) -> int:
+        """Long rows whose symbolic column bitmap the plan keeps."""
+        n = C.c_int64()
+        check(lib().ops_plan_kept_rows(self.ptr, C.byref(n)))
+        return int(n.value)
+
     def rowptr64(self, stream=None) -> torch.Tensor:
         if self._rpt64 is None:
             t = torch.empty(self.M + 1, dtype=torch.int64, device="cuda")
@@ -333,7 +362,10 @@ class Plan:
 
 def backward(A: DeviceCSR, B: DeviceCSR, crpt64: torch.Tensor, ccol: torch.Tensor, dc: torch.Tensor,
              ws: Workspace | None = None, stream=None):
-    """(dA values on pattern(A), dB values on pattern(B)) for L with dL/dC = dc."""
+    """(dA values on pattern(A), dB values on pattern(B)) for L with dL/dC = dc.
+
+    Each has one value per entry of its descriptor, from rpt[0] on (a row
+    slice's gradients are the slice's nnz values)."""
     ws = ws or default_workspace()
     dt = A.val.dtype
     da = torch.empty(max(A.nnz, 1), dtype=dt, device="cuda")
@@ -350,8 +382,11 @@ class SpGEMMFunction(torch.autograd.Function):
     """C_values = spgemm(A, B) on fixed patterns, differentiable in the
     values of A and B.  Backward is the masked pair
     dA = (dC B^T) on pattern(A), dB = (A^T dC) on pattern(B), computed by
-    the library's HIP backward kernels (C's structure is recomputed from
-    the plan, no hash tables are kept)."""
+    the library's HIP backward kernels.  The backward reads C's structure
+    from the plan (row offsets) and the forward's column list; no hash
+    tables are stored between the passes.  The plan itself holds device
+    memory until it dies, including kept long-row column bitmaps within
+    the process-wide limit (set_kept_bitmap_limit)."""
 
     @staticmethod
     def forward(ctx, a_val, b_val, A: DeviceCSR, B: DeviceCSR, plan: Plan):
@@ -379,4 +414,5 @@ def spgemm_autograd(a_val: torch.Tensor, b_val: torch.Tensor, A: DeviceCSR, B: D
 
 __all__ = ["DeviceCSR", "Workspace", "LibCSR", "Plan", "spgemm", "spgemm_safe", "backward", "SpGEMMFunction",
            "spgemm_autograd", "default_workspace", "OpsTimings", "OpsError", "dtype_code", "batch_pack",
-           "default_tuning", "tuning_caps", "mem_cache_stats", "set_mem_cache_limit", "mem_cache_release"]
+           "default_tuning", "tuning_caps", "mem_cache_stats", "set_mem_cache_limit", "mem_cache_release",
+           "set_kept_bitmap_limit"]
