@@ -1,17 +1,21 @@
 """SpGEMM benchmark on one MI355X.
 
-Default workload: C = A*A for the 27-point stencil on a 100^3 grid (1M rows,
-FP64, seed 1).  A step is one drop-in call (ops_spgemm_f64: setup, binning,
-symbolic, scan, allocation of C, numeric) with inputs resident in HBM; C is
-released outside the timed region, as the reference releases it outside its
-timed call (benchmarks/opsparse_profile.cu:24-39).
+Default workload: C = A*A for the R-MAT 2^20 matrix (1,048,576 rows,
+(a,b,c,d) = (.57,.19,.19,.05), 16 * 2^20 edges merged, FP64, seed 2), the
+largest configuration that fits one GPU: 2.09e10 products and 9.7e9 output
+entries, so C takes 64-bit row offsets (ops_spgemm64_f64).  A step is one
+call (setup, binning, symbolic, scan, allocation of C, numeric) with inputs
+resident in HBM; C is released outside the timed region, as the reference
+releases it outside its timed call (benchmarks/opsparse_profile.cu:24-39).
+The 27-point stencil on 100^3 (FP64, seed 1) is measured after it and kept
+as a nested record (config.stencil_100).
 
 Other workloads (--workload):
+  stencil  : the stencil alone
   uniform  : 4096^2, 8 distinct uniform columns per row, FP64, seed 0
-  rmat20   : R-MAT 2^20 A*A, FP64, seed 2, 64-bit row offsets (nnz(C) > 2^31)
-  batch    : 4096 independent C_i = A_i*B_i packed block-diagonally, one fused
-             autotuned call (and a replayed numeric hipGraph) against the plain
-             call looped over the instances; FP64 line, FP32 in the config
+  batch    : 4096 independent C_i = A_i*B_i, one fused autotuned call (and
+             a replayed whole-call hipGraph) against the plain call looped
+             over the instances; FP64 line, FP32 in the config
   backward : R-MAT 2^18 A*B in FP32, forward + masked backward, gradients
              checked against FP64 direct sums and FP64 finite differences
 
@@ -25,8 +29,10 @@ timed calls record no events), with its algorithmic bytes per launch:
 (P products, Z output entries, nnzA A-nonzeros and rows of that bin; w the
 value size).  traffic = measured HBM bytes per launch of that kernel when a
 counter summary for it is committed under profiles/.  cpu_baseline = the CPU
-oracle (OpenMP Gustavson) timed on this host's cores over the workload or a
-stated row sample of it; the same oracle output checks the GPU rows.
+oracle (OpenMP Gustavson, threads bound with OMP_PROC_BIND=spread) timed on
+this host's cores over the workload or a stated row sample of it; the same
+oracle output checks the GPU rows.  cold_* = one call with the output cache
+emptied (and with a new workspace), the reference's per-call allocation.
 """
 from __future__ import annotations
 
@@ -38,6 +44,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baseline's OpenMP threads are spread over the host's cores; set
+# before any OpenMP runtime in this process starts.
+os.environ.setdefault("OMP_PROC_BIND", "spread")
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "SpGEMM GFLOP/s (2×intermediate products/s), % HBM roofline; batched & fwd+bwd"
@@ -48,7 +57,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="stencil", choices=["stencil", "uniform", "rmat20", "batch", "backward"])
+    p.add_argument("--workload", default="rmat20", choices=["stencil", "uniform", "rmat20", "batch", "backward"])
+    p.add_argument("--no-stencil", action="store_true", help="rmat20: skip the nested stencil record")
     p.add_argument("--cpu-iters", type=int, default=6)
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle (timing and row checks)")
     p.add_argument("--no-check", action="store_true", help="skip output checks")
@@ -170,12 +180,13 @@ def cpu_leg(A, B, r0, r1, iters, desc, gpu_c, tol, check):
 
 def phase_ms(tims, host=None):
     """Mean host-side phase times (ms) of `host` (default: tims) and the last of `tims`' per-bin
-    kernel event times (in calls with events the symbolic kernels are waited for through their
-    events, so "scan" is the scan alone)."""
+    kernel event times.  The host phases are read in calls that do not synchronise between
+    phases, so "symbolic_and_scan" runs from the end of setup to the end of the scan (the
+    symbolic kernels and the scan, waited for together)."""
     import numpy as np
     host = host or tims
     mean = lambda f: round(float(np.mean([getattr(t, f) for t in host])) * 1e3, 3)
-    return {"setup": mean("setup"), "scan": mean("prefix"), "allocate_c": mean("allocate"),
+    return {"setup": mean("setup"), "symbolic_and_scan": mean("prefix"), "allocate_c": mean("allocate"),
             "numeric": mean("numeric"), "total": mean("total"),
             "symbolic_bin_ms": [round(x, 3) for x in tims[-1].sym_ms],
             "numeric_bin_ms": [round(x, 3) for x in tims[-1].num_ms],
@@ -218,8 +229,35 @@ def vendor_leg(dA, dB, dtype_f32, reps, ours_ms, ours_nnz):
             "speedup_ours_vs_rocsparse": round(ms / ours_ms, 2), "calls": reps}
 
 
+def cold_calls(dA, ws, wide):
+    """The first call's cost when its memory is not yet held, in ms: with
+    the output cache emptied (C's arrays come from hipMalloc, whose fresh
+    VRAM the driver clears, as the reference's per-call cudaMalloc of C does,
+    src/opsparse/core_algo/spgemm/spgemm.cu:44-45), and additionally with a
+    new workspace (its buffers allocated again, as the reference's
+    SpgemmMeta is in every call, meta.cu:16-48)."""
+    import torch
+    from opsparse_amd.spgemm import Workspace, mem_cache_release, spgemm
+    out = {}
+    for key, fresh_ws in (("cold_output_ms", False), ("cold_output_and_workspace_ms", True)):
+        torch.cuda.synchronize()
+        mem_cache_release()
+        w = Workspace() if fresh_ws else ws
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c = spgemm(dA, dA, ws=w, wide=wide)
+        torch.cuda.synchronize()
+        out[key] = round((time.perf_counter() - t0) * 1e3, 3)
+        c.free()
+        if fresh_ws:
+            del w
+    mem_cache_release()
+    return out
+
+
 def run_single(args, name):
-    """stencil / uniform / rmat20: one drop-in call per step."""
+    """stencil / uniform / rmat20: one drop-in call per step.  Returns
+    (GFLOP/s, ms per call, config, roofline, cpu_baseline)."""
     import numpy as np
     import torch
     import opsparse_amd.gen as G
@@ -234,7 +272,7 @@ def run_single(args, name):
         desc = "C=A*A, A = 4096x4096, 8 uniform distinct columns per row, U[-1,1], FP64, seed 0"
     else:
         A = G.rmat(20, 16, 2)
-        desc = ("C=A*A, A = R-MAT 2^20, (a,b,c,d)=(.57,.19,.19,.05), 16*2^20 edges merged, no vertex "
+        desc = ("C=A*A, A = R-MAT 2^20 x 2^20, (a,b,c,d)=(.57,.19,.19,.05), 16*2^20 edges merged, no vertex "
                 "permutation, U[-1,1], FP64, seed 2, 64-bit row offsets")
     P = G.compute_flop_host(A, A)
     t_gen = time.time() - t_gen
@@ -260,6 +298,7 @@ def run_single(args, name):
            "pipeline_roofline_frac": round(bytes_alg / t_step / 1e9 / HBM_PEAK_GBS, 4),
            "bytes_alg": int(bytes_alg), "compulsory_bytes": int(bytes_comp),
            "compulsory_roofline_frac": round(bytes_comp / t_step / 1e9 / HBM_PEAK_GBS, 4),
+           "kept_bitmap_rows": int(tims[-1].kept_rows),
            "phases": phase_ms(tims, timed), "parallelism": "single GPU"}
     if args.vendor or (name != "rmat20" and not args.no_vendor):
         torch.cuda.synchronize()
@@ -278,14 +317,33 @@ def run_single(args, name):
                     ref = oracle.spgemm_rows(A, A, s0, s0 + 4096)
                     rpt, col, val = c_last.rows_to_host(s0, s0 + 4096)
                     worst = max(worst, oracle.compare(rpt, col, val, ref, 1e-12)["max_norm"])
-                cfg["parity"] = {"structure": "exact on rows 0..1023 and 4 blocks of 4096 rows",
-                                 "max_norm_err": worst}
+                # every row offset against the oracle's symbolic count
+                t0 = time.time()
+                rows_exact = bool(np.array_equal(c_last.rowptr_to_host(), oracle.symbolic(A, A)))
+                cfg["parity"] = {"structure": "row offsets of all rows " + ("exact" if rows_exact else "DIFFER")
+                                              + "; columns exact on rows 0..1023 and 4 blocks of 4096 rows",
+                                 "max_norm_err": worst, "row_offsets_check_s": round(time.time() - t0, 1)}
+                assert rows_exact, "row offsets differ from the oracle"
         else:
             cpu, err = cpu_leg(A, A, 0, A.M, args.cpu_iters, "full workload", c_last, 1e-12, not args.no_check)
             if err is not None:
                 cfg["parity"] = {"structure": "exact", "max_norm_err": err["max_norm"], "max_rel_err": err["max_rel"]}
     c_last.free()
-    return line(gflops, args.steps, args.warmup, ms_per_step, "f64", cfg, roof, cpu)
+    cfg.update(cold_calls(dA, ws, wide))
+    log(f"{name}: cold calls {cfg['cold_output_ms']} / {cfg['cold_output_and_workspace_ms']} ms")
+    ws.trim()
+    return gflops, ms_per_step, cfg, roof, cpu
+
+
+def nested_record(gflops, ms, cfg, roof, cpu):
+    """A secondary workload's figures, kept inside the headline's config."""
+    keep = ("workload", "products", "nnz_c", "pipeline_roofline_frac", "compulsory_roofline_frac",
+            "cold_output_ms", "cold_output_and_workspace_ms", "parity", "vendor_rocsparse", "phases")
+    rec = {"gflops": round(gflops, 3), "ms_per_call": round(ms, 4)}
+    rec.update({k: cfg[k] for k in keep if k in cfg})
+    rec["roofline"] = roof
+    rec["cpu_baseline"] = cpu
+    return rec
 
 
 # ----------------------------------------------------------------------------- batched
@@ -390,122 +448,6 @@ def run_batch(args):
 
 # ----------------------------------------------------------------------------- forward + backward
 
-def _gather_rows(rpt, rows):
-    """Indices of the entries of `rows` (in that order) of a CSR with row offsets rpt."""
-    import numpy as np
-    starts, ends = rpt[rows], rpt[rows + 1]
-    lens = ends - starts
-    tot = int(lens.sum())
-    first = np.cumsum(lens) - lens
-    return np.repeat(starts - first, lens) + np.arange(tot, dtype=np.int64)
-
-
-def gradient_checks(A, B, dA64, dB64, crpt_h, ccol_h, dc_h, ga32, gb32, ga64, gb64, n_entries, ws):
-    """FP64 checks of sampled gradient entries: direct sums on the host
-    (dA[i,k] = sum_j dC[i,j] B[k,j], dB[k,j] = sum_i A[i,k] dC[i,j]) and
-    central differences of L = <dC, A*B> through the GPU forward, restricted
-    to the rows of C an entry feeds.  Errors are normalised by the sum of
-    |terms| of the entry's gradient."""
-    import numpy as np
-    import torch
-    from opsparse_amd.gen import HostCSR
-    from opsparse_amd.spgemm import DeviceCSR, spgemm
-    rng = np.random.default_rng(4)
-    a_rows = np.repeat(np.arange(A.M, dtype=np.int64), np.diff(A.rpt.astype(np.int64)))
-    b_rows = np.repeat(np.arange(B.M, dtype=np.int64), np.diff(B.rpt.astype(np.int64)))
-    order = np.argsort(A.col, kind="stable")  # A by column: rows feeding each row of B
-    cstart = np.zeros(A.N + 1, dtype=np.int64)
-    np.cumsum(np.bincount(A.col, minlength=A.N), out=cstart[1:])
-    ga32_h, gb32_h = ga32.double().cpu().numpy(), gb32.double().cpu().numpy()
-    ga64_h, gb64_h = ga64.cpu().numpy(), gb64.cpu().numpy()
-    # L is linear in each entry, so the central difference is exact up to
-    # rounding for any step; a large step keeps the rounding of the other
-    # terms of the affected rows (which do not scale with it) negligible.
-    h = 1024.0
-
-    def sub_csr(rows, k=None, j_local=None, delta=0.0):
-        idx = _gather_rows(A.rpt.astype(np.int64), rows)
-        rpt = np.zeros(len(rows) + 1, dtype=np.int32)
-        np.cumsum(np.diff(A.rpt.astype(np.int64))[rows], out=rpt[1:])
-        val = A.val[idx].astype(np.float64)
-        if j_local is not None:
-            val[j_local] += delta
-        return HostCSR(len(rows), A.N, rpt, A.col[idx].astype(np.int32), val)
-
-    def c_rows(Ahost, Bdev):
-        dAh = DeviceCSR.from_host(Ahost, torch.float64)
-        c = spgemm(dAh, Bdev, ws=ws, wide=True)
-        rpt, col, val = c.to_host()
-        c.free()
-        return rpt, col, val
-
-    out = {"fd64": 0.0, "ref64": 0.0, "ref32": 0.0, "zero_entries": 0, "zero_max_abs_fd": 0.0,
-           "zero_max_abs_grad": 0.0}
-
-    def record(fd, ref, scale, g64, g32):
-        if scale == 0.0:  # no terms: the gradient is structurally zero
-            out["zero_entries"] += 1
-            out["zero_max_abs_fd"] = max(out["zero_max_abs_fd"], abs(fd))
-            out["zero_max_abs_grad"] = max(out["zero_max_abs_grad"], abs(g64), abs(g32))
-            return
-        out["fd64"] = max(out["fd64"], abs(fd - g64) / scale)
-        out["ref64"] = max(out["ref64"], abs(ref - g64) / scale)
-        out["ref32"] = max(out["ref32"], abs(ref - g32) / scale)
-    n_half = n_entries // 2
-    for cnt, e in enumerate(rng.choice(A.nnz, n_half, replace=False)):
-        if cnt % 128 == 0:
-            log(f"gradient check: dA entry {cnt}/{n_half}")
-        i, k = int(a_rows[e]), int(A.col[e])
-        crow = ccol_h[crpt_h[i]:crpt_h[i + 1]]
-        bj = B.col[B.rpt[k]:B.rpt[k + 1]]
-        bv = B.val[B.rpt[k]:B.rpt[k + 1]].astype(np.float64)
-        pos = np.searchsorted(crow, bj)
-        assert np.array_equal(crow[pos], bj), "pattern(C) misses a product of a row of A"
-        dcr = dc_h[crpt_h[i] + pos]
-        ref, scale = float(np.dot(dcr, bv)), float(np.dot(np.abs(dcr), np.abs(bv)))
-        # FD: row i with A[i,k] +- h, both in one call
-        local = e - int(A.rpt[i])
-        Hp = sub_csr(np.array([i]), j_local=local, delta=h)
-        Hm = sub_csr(np.array([i]), j_local=local, delta=-h)
-        step = float(Hp.val[local]) - float(Hm.val[local])
-        two = HostCSR(2, A.N, np.concatenate([Hp.rpt, Hp.rpt[1:] + Hp.rpt[-1]]).astype(np.int32),
-                      np.concatenate([Hp.col, Hm.col]), np.concatenate([Hp.val, Hm.val]))
-        rpt, col, val = c_rows(two, dB64)
-        n = int(rpt[1])
-        assert np.array_equal(col[:n], crow) and np.array_equal(col[n:], crow)
-        fd = float(np.dot(dc_h[crpt_h[i]:crpt_h[i + 1]], val[:n] - val[n:])) / step
-        record(fd, ref, scale, ga64_h[e], ga32_h[e])
-    for cnt, e in enumerate(rng.choice(B.nnz, n_entries - n_half, replace=False)):
-        if cnt % 128 == 0:
-            log(f"gradient check: dB entry {cnt}/{n_entries - n_half}")
-        k, j = int(b_rows[e]), int(B.col[e])
-        feed = order[cstart[k]:cstart[k + 1]]
-        rows = a_rows[feed]
-        av = A.val[feed].astype(np.float64)
-        dcv = np.empty(len(rows))
-        for t, r in enumerate(rows):
-            crow = ccol_h[crpt_h[r]:crpt_h[r + 1]]
-            p = int(np.searchsorted(crow, j))
-            assert p < crow.size and crow[p] == j
-            dcv[t] = dc_h[crpt_h[r] + p]
-        ref, scale = float(np.dot(av, dcv)), float(np.dot(np.abs(av), np.abs(dcv)))
-        fd = 0.0
-        if len(rows):
-            sub = sub_csr(rows)
-            res, pert = [], []
-            for s in (h, -h):
-                bval = dB64.val.clone()
-                bval[int(e)] += s
-                pert.append(float(bval[int(e)]))
-                res.append(c_rows(sub, dB64.with_values(bval)))
-            (rp, cp, vp), (_, cm, vm) = res
-            idx = _gather_rows(crpt_h, rows)
-            assert np.array_equal(cp, ccol_h[idx]) and np.array_equal(cm, cp)
-            fd = float(np.dot(dc_h[idx], vp - vm)) / (pert[0] - pert[1])
-        record(fd, ref, scale, gb64_h[e], gb32_h[e])
-    return out
-
-
 def run_backward(args):
     """R-MAT 2^18 A*B in FP32: forward (drop-in call) + masked backward."""
     import numpy as np
@@ -567,6 +509,7 @@ def run_backward(args):
         ga64, gb64 = backward(dA64, dB64, rpt, col, dc64, ws=ws)
         torch.cuda.synchronize()
         t0 = time.time()
+        from opsparse_amd.gradcheck import gradient_checks
         chk = gradient_checks(A, B, dA64, dB64, crpt_h, ccol_h, dc64.cpu().numpy(), ga, gb, ga64, gb64,
                               args.fd_entries, ws)
         cfg["gradient_check"] = {
@@ -610,7 +553,13 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.workload in ("stencil", "uniform", "rmat20"):
-        out = run_single(args, args.workload)
+        gflops, ms, cfg, roof, cpu = run_single(args, args.workload)
+        if args.workload == "rmat20" and not args.no_stencil:
+            # the 27-point stencil (1M rows, LDS-hash bins only), as a nested record
+            from opsparse_amd.spgemm import mem_cache_release
+            mem_cache_release()
+            cfg["stencil_100"] = nested_record(*run_single(args, "stencil"))
+        out = line(gflops, args.steps, args.warmup, ms, "f64", cfg, roof, cpu)
     elif args.workload == "batch":
         out = run_batch(args)
     else:

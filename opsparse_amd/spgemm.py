@@ -208,6 +208,15 @@ class LibCSR:
                     raise RuntimeError(f"hipMemcpy failed: {st}")
         return rpt.astype(np.int64), col, val
 
+    def rowptr_to_host(self):
+        """All M+1 row offsets as an int64 numpy array (columns and values stay on the device)."""
+        ridx = np.int64 if self.wide else np.int32
+        rpt = np.empty(self.M + 1, dtype=ridx)
+        torch.cuda.synchronize()
+        if _hip().hipMemcpy(rpt.ctypes.data, self.d.rpt, rpt.nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        return rpt.astype(np.int64)
+
     def rows_to_host(self, r0: int, r1: int):
         """Rows [r0, r1) as (rpt rebased to 0, int64; col; val) numpy copies,
         reading only that slice of the device arrays."""
@@ -272,6 +281,19 @@ def spgemm(A: DeviceCSR, B: DeviceCSR, ws: Workspace | None = None, timings: Ops
         fn = lib().ops_spgemm_f64 if f64 else lib().ops_spgemm_f32
     check(fn(ws.ptr, C.byref(a), C.byref(b), C.byref(c), _stream_ptr(stream), t), ws.ptr)
     return LibCSR(c, wide, A.val.dtype)
+
+
+def compute_flop_device(A: DeviceCSR, B: DeviceCSR, ws: Workspace | None = None, stream=None):
+    """Products per row of A*B on the device (ops_compute_flop_device):
+    (row_flop int32 tensor of A.M entries, saturated at INT32_MAX; 64-bit total)."""
+    _require_gpu()
+    ws = ws or default_workspace()
+    rf = torch.empty(max(A.M, 1), dtype=torch.int32, device="cuda")
+    tot = C.c_int64()
+    a, b = A.desc(), B.desc()
+    check(lib().ops_compute_flop_device(ws.ptr, C.byref(a), C.byref(b), rf.data_ptr(), C.byref(tot),
+                                        _stream_ptr(stream)), ws.ptr)
+    return rf[:A.M], int(tot.value)
 
 
 def spgemm_safe(A: DeviceCSR, B: DeviceCSR, A_host: HostCSR, B_host: HostCSR, ws: Workspace | None = None):
@@ -413,6 +435,6 @@ def spgemm_autograd(a_val: torch.Tensor, b_val: torch.Tensor, A: DeviceCSR, B: D
 
 
 __all__ = ["DeviceCSR", "Workspace", "LibCSR", "Plan", "spgemm", "spgemm_safe", "backward", "SpGEMMFunction",
-           "spgemm_autograd", "default_workspace", "OpsTimings", "OpsError", "dtype_code", "batch_pack",
+           "spgemm_autograd", "default_workspace", "OpsTimings", "OpsError", "dtype_code", "batch_pack", "compute_flop_device",
            "default_tuning", "tuning_caps", "mem_cache_stats", "set_mem_cache_limit", "mem_cache_release",
            "set_kept_bitmap_limit"]

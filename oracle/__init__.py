@@ -66,6 +66,14 @@ def compute_flop(A, B, per_row: bool = False):
     return (int(tot), rf) if per_row else int(tot)
 
 
+def symbolic(A, B):
+    """Row offsets of A*B alone: rpt int64[M+1] (exclusive sum of each row's
+    number of distinct columns), without computing columns or values."""
+    rpt = np.zeros(A.M + 1, dtype=np.int64)
+    lib().oracle_symbolic(A.M, B.N, _p(A.rpt), _p(A.col), _p(B.rpt), _p(B.col), _p(rpt))
+    return rpt
+
+
 def spgemm(A, B, with_abs: bool = True):
     """Returns (rpt int64[M+1], col int32, val float64, absum float64 or None)."""
     M, N = A.M, B.N

@@ -59,7 +59,7 @@ def test_backward_stencil_and_long_rows(gpu):
     N = 20000
     A = random_csr(24, N, np.r_[rng.integers(100, 160, 12), rng.integers(1, 4, 12)], rng)
     B = random_csr(N, N, rng.integers(50, 90, N), rng)
-    check_grad(A, B, torch.float64)  # rows of C above 4096 entries use the sorted-row search
+    check_grad(A, B, torch.float64)  # rows of C above 4096 entries use the bitmap-rank windows (k_bwd_rank)
 
 
 def test_autograd_function_and_finite_differences(gpu):
@@ -111,6 +111,50 @@ def test_backward_long_rows_skipped_windows(gpu):
     B = G.HostCSR(N, N, B.rpt, col, B.val)
     check_grad(A, B, torch.float64)
     check_grad(A, B, torch.float32)
+
+
+def test_backward_row_slice_descriptors(gpu):
+    """A and B given as row-slice descriptors (rpt not starting at 0, col and
+    val shared with a larger matrix): dA and dB hold one value per entry of
+    each slice, from the slice's first entry, and match the oracle on the
+    same slices rebased to 0.  Buffers are exactly the slices' nnz, with
+    guard values after them that must stay untouched."""
+    rng = np.random.default_rng(41)
+    big_a = random_csr(300, 500, np.r_[rng.integers(0, 40, 290), rng.integers(200, 300, 10)], rng)
+    big_b = random_csr(900, 60000, rng.integers(0, 90, 900), rng)
+    for dtype in (torch.float64, torch.float32):
+        npdt = np.float64 if dtype == torch.float64 else np.float32
+        Ah_full, Bh_full = big_a.astype(npdt), big_b.astype(npdt)
+        dA_full, dB_full = DeviceCSR.from_host(Ah_full, dtype), DeviceCSR.from_host(Bh_full, dtype)
+        dA, dB = dA_full.row_slice(40, 300), dB_full.row_slice(200, 700)
+        Ah, Bh = Ah_full.row_slice(40, 300), Bh_full.row_slice(200, 700)
+        assert dA.nnz == Ah.nnz and dB.nnz == Bh.nnz and int(dA.rpt[0]) > 0
+        plan = Plan(dA, dB)
+        rpt = plan.rowptr64()
+        col, _ = plan.numeric(dA, dB)
+        dc = torch.from_numpy(rng.normal(size=plan.nnz)).to(device="cuda", dtype=dtype)
+        from opsparse_amd.spgemm import dtype_code, _stream_ptr, default_workspace
+        from opsparse_amd._lib import check, lib
+        import ctypes as C
+        guard = 12345.0
+        da = torch.full((dA.nnz + 64,), guard, dtype=dtype, device="cuda")
+        db = torch.full((dB.nnz + 64,), guard, dtype=dtype, device="cuda")
+        a, b = dA.desc(), dB.desc()
+        ws = default_workspace()
+        check(lib().ops_spgemm_backward(ws.ptr, dtype_code(dtype), C.byref(a), C.byref(b), rpt.data_ptr(),
+                                        col.data_ptr(), dc.data_ptr(), da.data_ptr(), db.data_ptr(),
+                                        _stream_ptr()), ws.ptr)
+        torch.cuda.synchronize()
+        assert torch.all(da[dA.nnz:] == guard) and torch.all(db[dB.nnz:] == guard)
+        rpt_h, col_h, dc_h = rpt.cpu().numpy(), col.cpu().numpy(), dc.cpu().numpy()
+        ra, rb = oracle.backward(Ah, Bh, rpt_h, col_h, dc_h)
+        sa, sb = oracle.backward(absv(Ah), absv(Bh), rpt_h, col_h, np.abs(dc_h))
+        ea = np.abs(da[:dA.nnz].double().cpu().numpy() - ra) / np.maximum(sa, 1e-300)
+        eb = np.abs(db[:dB.nnz].double().cpu().numpy() - rb) / np.maximum(sb, 1e-300)
+        assert max(ea.max(initial=0.0), eb.max(initial=0.0)) <= TOL[dtype]
+        # the Python wrapper returns the same slice-sized gradients
+        ga, gb = backward(dA, dB, rpt, col, dc)
+        assert ga.numel() == dA.nnz and gb.numel() == dB.nnz
 
 
 def test_random_backward_and_plan_parity(gpu):

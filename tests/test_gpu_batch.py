@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 def row_slice(D: DeviceCSR, r0: int, r1: int) -> DeviceCSR:
     # descriptor of rows [r0, r1): rpt keeps absolute offsets into col/val
-    return DeviceCSR(r1 - r0, D.N, D.rpt[r0:r1 + 1], D.col, D.val)
+    return D.row_slice(r0, r1)
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
@@ -78,6 +78,60 @@ def test_batch_pack_matches_generator_packing(gpu):
     C = spgemm(pa, pb, wide=True)
     crpt, ccol, cval = C.to_host()
     oracle.compare(crpt, ccol, cval, oracle.spgemm(b.A, b.B), 1e-12)
+
+
+def test_batch_pack_row_slices_and_empty_instances(gpu):
+    """Packing descriptors that are row slices of larger matrices (rpt not
+    starting at 0; each instance's entry count is its own rpt[M] - rpt[0]),
+    instances with no rows (null rpt, never read) and a descriptor whose nnz
+    disagrees with its row offsets (rejected).  The packed product matches
+    the oracle instance by instance."""
+    from opsparse_amd._lib import OPS_INVALID, OpsError
+    from opsparse_amd.spgemm import batch_pack
+    rng = np.random.default_rng(31)
+    from helpers import random_csr
+    parents_a = [random_csr(400, 300, rng.integers(0, 20, 400), rng) for _ in range(3)]
+    parents_b = [random_csr(300, 500, rng.integers(0, 25, 300), rng) for _ in range(3)]
+    dpa = [DeviceCSR.from_host(X, torch.float64) for X in parents_a]
+    dpb = [DeviceCSR.from_host(X, torch.float64) for X in parents_b]
+    sl = [(0, 120), (150, 400), (37, 38)]
+    ha, hb, da, db = [], [], [], []
+    for j, (r0, r1) in enumerate(sl):
+        ha.append(parents_a[j].row_slice(r0, r1))
+        da.append(dpa[j].row_slice(r0, r1))
+        hb.append(parents_b[j])
+        db.append(dpb[j])
+    # an instance with no rows: A_1 is 0 x 0 and B_1 is 0 x 500 (rpt null)
+    empty_a = DeviceCSR(0, 0, torch.empty(0, dtype=torch.int32, device="cuda"),
+                        torch.empty(0, dtype=torch.int32, device="cuda"),
+                        torch.empty(0, dtype=torch.float64, device="cuda"), 0)
+    empty_b = DeviceCSR(0, 500, empty_a.rpt, empty_a.col, empty_a.val, 0)
+    assert empty_a.desc().rpt is None or empty_a.desc().rpt == 0
+    # instance order: slice 0, the empty one, slice 1, slice 2
+    order_a = [da[0], empty_a, da[1], da[2]]
+    order_b = [dpb[0].row_slice(0, 300), empty_b, dpb[1], dpb[2]]
+    pa, rba, cba = batch_pack(order_a)
+    pb, rbb, cbb = batch_pack(order_b)
+    assert list(rba) == [0, 120, 120, 370, 371] and list(cba) == [0, 300, 300, 600, 900]
+    assert list(rbb) == [0, 300, 300, 600, 900] and list(cbb) == [0, 500, 1000, 1500, 2000]
+    assert pa.nnz == sum(h.nnz for h in ha)
+    C = spgemm(pa, pb, wide=True)
+    crpt, ccol, cval = C.to_host()
+    inst_a = [ha[0], None, ha[1], ha[2]]
+    inst_b = [hb[0], None, hb[1], hb[2]]
+    for i in range(4):
+        r0, r1 = int(rba[i]), int(rba[i + 1])
+        s0, s1 = crpt[r0], crpt[r1]
+        if inst_a[i] is None:
+            assert r0 == r1
+            continue
+        ref = oracle.spgemm(inst_a[i], inst_b[i])
+        oracle.compare(crpt[r0:r1 + 1] - s0, ccol[s0:s1] - int(cbb[i]), cval[s0:s1], ref, 1e-12)
+    # nnz disagreeing with rpt[M] - rpt[0] is rejected
+    bad = DeviceCSR(da[1].M, da[1].N, da[1].rpt, da[1].col, da[1].val, da[1].nnz + 1)
+    with pytest.raises(OpsError) as ei:
+        batch_pack([da[0], bad])
+    assert ei.value.status == OPS_INVALID
 
 
 def test_every_tuning_candidate_is_exact(gpu):

@@ -343,15 +343,25 @@ def test_long_rows_kept_bitmaps(gpu, monkeypatch, N):
     A = random_csr(len(lens), K, lens, rng)
     B = random_csr(K, N, rng.integers(250, 350, K), rng)
     ref = oracle.spgemm(A, B)
+    from opsparse_amd._lib import OpsTimings
+    n_long = None
     for kept in ("", "0", "2"):
         if kept:
             monkeypatch.setenv("OPSAMD_KEPT_BITMAPS", kept)
         log(f"kept bitmaps N={N} limit={kept or 'none'}")
         run(A, B)
         run(A, B, np.float32)
-        # a plan keeps its own bitmaps for its eager and replayed numeric phases
         dA, dB = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+        t = OpsTimings()
+        spgemm(dA, dB, timings=t).free()
+        if not kept:
+            n_long = int(t.num_rows[7])
+            assert n_long >= 3 and t.kept_rows == n_long, (list(t.num_rows), t.kept_rows)
+        else:
+            assert t.kept_rows == min(int(kept), n_long), (kept, t.kept_rows)
+        # a plan keeps its own bitmaps for its eager and replayed numeric phases
         p = Plan(dA, dB)
+        assert p.kept_rows == t.kept_rows
         rpt = p.rowptr64().cpu().numpy()
         col, val = p.numeric(dA, dB)
         oracle.compare(rpt, col.cpu().numpy(), val.cpu().numpy(), ref, TOL[np.float64])

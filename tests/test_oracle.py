@@ -145,6 +145,27 @@ def test_stencil_square_known_answer():
         assert rpt[i + 1] - rpt[i] == cnt
 
 
+def test_symbolic_matches_scipy_pattern_and_closed_form():
+    """oracle.symbolic (row offsets only) against scipy on all-ones values
+    (no cancellation: the pattern is the structural one) and against the
+    stencil's per-row closed form."""
+    rng = np.random.default_rng(19)
+    A = random_csr(400, 300, rng.integers(0, 30, 400), rng)
+    B = random_csr(300, 5000, rng.integers(0, 80, 300), rng)
+    ones = lambda X: sp.csr_matrix((np.ones(X.nnz), X.col, X.rpt), shape=(X.M, X.N))
+    P = ones(A) @ ones(B)
+    assert np.array_equal(oracle.symbolic(A, B), P.indptr.astype(np.int64))
+    n = 9
+    S = G.stencil27(n, 2)
+    rpt = oracle.symbolic(S, S)
+    for i in range(S.M):
+        x, y, z = i % n, (i // n) % n, i // (n * n)
+        cnt = 1
+        for c in (x, y, z):
+            cnt *= min(c + 2, n - 1) - max(c - 2, 0) + 1
+        assert rpt[i + 1] - rpt[i] == cnt
+
+
 def test_rows_matches_full():
     A = G.rmat(10, 8, 3)
     full = oracle.spgemm(A, A)
