@@ -172,7 +172,10 @@ def test_rmat18_forward_backward_fd(gpu):
     chk = gradient_checks(A, B, dA64, dB64, crpt_h, ccol_h, dc_h, ga, gb, ga64, gb64, 1024, ws)
     log(f"rmat18 gradient checks: {chk}")
     assert chk["fd64"] <= 1e-10 and chk["ref64"] <= 1e-12 and chk["ref32"] <= 1e-5
-    assert chk["zero_max_abs_fd"] == 0.0 and chk["zero_max_abs_grad"] == 0.0
+    # entries with no terms (an empty B row, or a B row no A column feeds):
+    # the gradient is exactly 0; the difference is 0 up to the rounding of
+    # the two perturbed rows' other terms, summed in different orders
+    assert chk["zero_max_abs_grad"] == 0.0 and chk["zero_max_abs_fd"] <= 1e-12
     # oracle dA on the 4096 highest-degree rows (those rows of dA depend on
     # those rows of C only)
     s0, s1 = int(crpt_h[0]), int(crpt_h[4096])
