@@ -140,6 +140,28 @@ int ops_spgemm_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_cs
 int ops_spgemm64_f64(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr64 *C, void *stream, ops_timings *t);
 int ops_spgemm64_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_csr64 *C, void *stream, ops_timings *t);
 
+/* C = A * B into caller-owned device arrays, allocating no device memory:
+ * no hipMalloc, hipFree or allocating library routine.  No counterpart in
+ * the reference; for callers that cannot allocate (serving loops with
+ * preallocated buffers, harnesses that forbid allocation; it synchronises
+ * the host, so it cannot itself be captured in a graph).  Variant of
+ * ops_spgemm64_f64 / _f32 (dt selects the value
+ * type) that takes no workspace: its scratch is a fixed array of about
+ * 21 MB inside the library's code object on each GPU, held until the
+ * process ends.  Limits: A has at most 262,144 rows and its last entry
+ * index is below 1,048,576 (A->rpt[M] <= 2^20); larger inputs return
+ * OPS_INVALID before any kernel runs.  The caller passes C->rpt (M+1 int64),
+ * C->col and C->val with room for `capacity` entries; on return C->M, C->N
+ * and C->nnz are set.  When nnz(C) > capacity the call returns OPS_INVALID
+ * with C->nnz = nnz(C), C->rpt filled and C->col / C->val untouched
+ * (retry with larger arrays).  All bins run on `stream`; the call synchronises the host
+ * twice and returns when C is complete.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (from several host threads
+ * or streams) race and must be serialised by the caller.  Long rows take
+ * the cursor path (no kept column bitmaps, no window-cut table). */
+int ops_spgemm64_noalloc(ops_dtype dt, const ops_csr *A, const ops_csr *B, ops_csr64 *C, int64_t capacity,
+                         void *stream);
+
 /* Releases library-allocated C arrays and zeroes the descriptor (CSR::drelease, csr.cu:40-48). */
 int ops_csr_free(ops_csr *C);
 int ops_csr64_free(ops_csr64 *C);

@@ -102,6 +102,7 @@ SIGNATURES = {
     "ops_spgemm_f32": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _P(OpsCsr), _vp, _P(OpsTimings)]),
     "ops_spgemm64_f64": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _P(OpsCsr64), _vp, _P(OpsTimings)]),
     "ops_spgemm64_f32": (C.c_int, [_vp, _P(OpsCsr), _P(OpsCsr), _P(OpsCsr64), _vp, _P(OpsTimings)]),
+    "ops_spgemm64_noalloc": (C.c_int, [C.c_int, _P(OpsCsr), _P(OpsCsr), _P(OpsCsr64), C.c_int64, _vp]),
     "ops_csr_free": (C.c_int, [_P(OpsCsr)]),
     "ops_csr64_free": (C.c_int, [_P(OpsCsr64)]),
     "ops_free": (C.c_int, [_vp]),

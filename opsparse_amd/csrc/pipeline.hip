@@ -137,6 +137,10 @@ struct ops_workspace {
     BinBounds bounds = default_bounds();
     int short_hash = kHashLin;
     bool sym_bitmap = false;
+    // Set only on the stack workspace of the allocation-free entry point:
+    // every bin runs on the caller's stream, and neither kept column
+    // bitmaps nor a window-cut table are made (both would allocate).
+    bool no_alloc = false;
     std::string err;
 };
 
@@ -201,6 +205,15 @@ void dispatch_bins(ops_workspace *ws, hipStream_t s, const int *size, bool timin
     int used = 0;
     for (int b = 0; b < kNumBins; b++) used += size[b] > 0 ? 1 : 0;
     if (used == 0) return;
+    if (ws->no_alloc) {
+        // one stream, no events (the workspace has none)
+        for (int b = kNumBins - 1; b >= 0; b--) {
+            if (size[b] <= 0) continue;
+            launch(b, s);
+            CK(hipGetLastError());
+        }
+        return;
+    }
     // The fork is recorded before the first launch, so the pool streams do
     // not wait for the bin that runs on s itself.
     if (used > 1) CK(hipEventRecord(ws->ev_fork, s));
@@ -326,7 +339,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     // ranges are known without a mark walk (one window, or the cut table).
     SymSave save;
     const int long_rows = st.sym_size[kNumBins - 1];
-    if (long_rows > 0 && B.N <= kSymLongWindow &&
+    if (long_rows > 0 && B.N <= kSymLongWindow && !ws->no_alloc &&
         (rank_windows(B.N) == 1 || cut_table_wanted(B.M, B.N))) {
         const long long words = (static_cast<long long>(B.N) + 31) / 32;
         size_t free_b = 0, total_b = 0;
@@ -481,14 +494,14 @@ EntryRange entry_range(const ops_csr *X, hipStream_t s)
 // (built here on stream s when B has more than one column window and the
 // table stays within kMaxCutBytes; it depends on B's pattern only).
 void numeric_scratch(PlanCore &pc, const ops_csr *A, const ops_csr *B, DevBuf<int> &cursor, DevBuf<int> &spill,
-                     DevBuf<int> &wcut, hipStream_t s)
+                     DevBuf<int> &wcut, hipStream_t s, bool allow_cut = true)
 {
     if (pc.num_size[kNumBins - 1] > 0) {
         pc.ls.span = static_cast<int>(a_index_span(A, s));
         cursor.ensure(4 * static_cast<size_t>(pc.ls.span) + 4);
         pc.ls.cursor = cursor.p;
         const int nwin = rank_windows(B->N);
-        if (cut_table_wanted(B->M, B->N)) {
+        if (allow_cut && cut_table_wanted(B->M, B->N)) {
             wcut.ensure(static_cast<size_t>(B->M) * (nwin + 1));
             launch_window_cuts(as_in(B), nwin, wcut.p, s);
             CK(hipGetLastError());
@@ -651,6 +664,104 @@ void backward_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const 
         launch_backward_bin<V>(bin, a, b, rows ? rows + off[bin] : nullptr, st.num_size[bin], crpt, ccol, dc, da, db,
                                a0, b0, cursor, span, ps);
     });
+}
+
+}  // namespace
+
+// ---- allocation-free entry point (ops_spgemm64_noalloc) -------------------
+//
+// Its scratch (row products, row nnz, bin lists, the spill list, the scan's
+// block sums, the counter block and the long-row cursors) lives in the
+// file-scope array below, which the code object brings with it: every GPU
+// the library runs on holds one copy until the process ends.  Limits: A
+// with at most kNaMaxRows rows and at most kNaMaxSpan entries (index of its
+// last entry + 1).
+constexpr int kNaMaxRows = 1 << 18;
+constexpr int kNaMaxSpan = 1 << 20;
+constexpr size_t kNaAlign = 256;
+constexpr size_t na_round(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr size_t kNaRowBytes = na_round(sizeof(int) * (static_cast<size_t>(kNaMaxRows) + 1));
+constexpr size_t kNaSumEntries = 256;  // >= scan_blocks(kNaMaxRows) + 2
+constexpr size_t kNaOffFlop = 0;
+constexpr size_t kNaOffNnz = kNaOffFlop + kNaRowBytes;
+constexpr size_t kNaOffBins = kNaOffNnz + kNaRowBytes;
+constexpr size_t kNaOffSpill = kNaOffBins + kNaRowBytes;
+constexpr size_t kNaOffSums = kNaOffSpill + kNaRowBytes;
+constexpr size_t kNaOffStats = kNaOffSums + na_round(sizeof(long long) * kNaSumEntries);
+constexpr size_t kNaOffCursor = kNaOffStats + na_round(sizeof(DevStats));
+constexpr size_t kNaBytes = kNaOffCursor + na_round(sizeof(int) * (4 * static_cast<size_t>(kNaMaxSpan) + 4));
+
+static __device__ __align__(256) unsigned long long scratch[kNaBytes / sizeof(unsigned long long)];
+
+namespace {
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (the array's name in host code is only a
+// stand-in for the symbol, not an address).
+std::mutex g_na_mu;
+unsigned char *g_na_ptr[kMaxDev] = {};
+
+unsigned char *na_scratch(int dev)
+{
+    if (dev < 0 || dev >= kMaxDev) throw HipFail(OPS_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        CK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_na_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    return g_na_ptr[dev];
+}
+
+template <typename T>
+void na_view(DevBuf<T> &b, unsigned char *base, size_t off, size_t bytes)
+{
+    b.p = reinterpret_cast<T *>(base + off);
+    b.cap = bytes / sizeof(T);
+}
+
+template <typename V>
+int spgemm_noalloc_impl(const ops_csr *A, const ops_csr *B, ops_csr64 *C, int64_t capacity, hipStream_t s)
+{
+    validate(A, B);
+    if (!C || (capacity > 0 && (!C->rpt || !C->col || !C->val)) || (A->M > 0 && !C->rpt) || capacity < 0)
+        throw HipFail(OPS_INVALID, "output arrays missing");
+    if (A->M > kNaMaxRows) throw HipFail(OPS_INVALID, "A has more rows than the allocation-free limit (262144)");
+    const EntryRange ea = entry_range(A, s);
+    if (ea.end > kNaMaxSpan)
+        throw HipFail(OPS_INVALID, "A's entries exceed the allocation-free limit (1048576 entry indices)");
+    init_attrs();
+    int dev = 0;
+    CK(hipGetDevice(&dev));
+    unsigned char *base = na_scratch(dev);
+    // A workspace on the stack whose buffers are views of the scratch (each
+    // sized to the limits above, so none ever grows) and which has no
+    // streams, events or pinned memory.
+    ops_workspace ws;
+    ws.device = dev;
+    ws.no_alloc = true;
+    DevStats h_stats{};
+    ws.d_stats = reinterpret_cast<DevStats *>(base + kNaOffStats);
+    ws.h_stats = &h_stats;
+    na_view(ws.row_flop, base, kNaOffFlop, kNaRowBytes);
+    na_view(ws.row_nnz, base, kNaOffNnz, kNaRowBytes);
+    na_view(ws.bins, base, kNaOffBins, kNaRowBytes);
+    na_view(ws.spill, base, kNaOffSpill, kNaRowBytes);
+    na_view(ws.block_sums, base, kNaOffSums, sizeof(long long) * kNaSumEntries);
+    na_view(ws.cursor, base, kNaOffCursor, kNaBytes - kNaOffCursor);
+    const CsrIn a = as_in(A), b = as_in(B);
+    const int M = A->M;
+    C->M = M;
+    C->N = B->N;
+    C->nnz = 0;
+    PlanCore pc;
+    symbolic_phase(&ws, a, b, pc, reinterpret_cast<long long *>(C->rpt), nullptr, ws.bins.p, s, nullptr, ws.kb);
+    C->nnz = pc.nnz;
+    if (pc.nnz > capacity) throw HipFail(OPS_INVALID, "nnz(C) exceeds the output capacity (C->nnz holds it)");
+    numeric_scratch(pc, A, B, ws.cursor, ws.spill, ws.wcut, s, false);
+    numeric_phase<V>(&ws, a, b, pc, C->col, static_cast<V *>(C->val), s, nullptr);
+    CK(hipStreamSynchronize(s));
+    return OPS_OK;
 }
 
 }  // namespace
@@ -854,6 +965,18 @@ int ops_spgemm64_f32(ops_workspace *ws, const ops_csr *A, const ops_csr *B, ops_
     if (!ws) return OPS_INVALID;
     return guarded(ws, [&]() {
         return spgemm_impl<float>(ws, A, B, nullptr, C, static_cast<hipStream_t>(stream), t);
+    });
+}
+
+int ops_spgemm64_noalloc(ops_dtype dt, const ops_csr *A, const ops_csr *B, ops_csr64 *C, int64_t capacity,
+                         void *stream)
+{
+    if (dt != OPS_F64 && dt != OPS_F32) return OPS_INVALID;
+    // Errors are reported through the status only (no workspace to hold a message).
+    return guarded(nullptr, [&]() {
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        return dt == OPS_F64 ? spgemm_noalloc_impl<double>(A, B, C, capacity, s)
+                             : spgemm_noalloc_impl<float>(A, B, C, capacity, s);
     });
 }
 
