@@ -678,6 +678,208 @@ __device__ __forceinline__ bool block_for_each_product(int a_beg, int a_end, con
     return a_end - a_beg <= BS;
 }
 
+// ---- Chunk walk: every 64-lane step inside one B range ---------------------
+//
+// The block walk above flattens a batch's products, so a step may straddle
+// several ranges and each such step pays an owner search.  For long-row
+// inputs most products sit in long ranges (R-MAT 2^20, rows above 4096
+// entries: 84% of a window's products come from ranges of 64 or more), and
+// there a step that never leaves its range is much cheaper: a range of len
+// products is cut into ceil(len / 64) chunks of one step each, the block's
+// waves take contiguous runs of chunks, and a chunk's B indices, A value
+// and A-nonzero are uniform values kept in scalar registers.  Lanes past a
+// range's end idle (about 20% of lanes on those windows), which costs far
+// less than the per-step ownership work it replaces.
+
+template <int BS, typename V>
+struct ChunkStage {
+    static constexpr int NW = BS / kWave;
+    struct Ent {
+        int bs;   // B index of the range's first product
+        int len;  // products in the range (> 0)
+        int e;    // A-nonzero index
+    };
+    V av[BS];         // A value of each range (0 when the walk has none)
+    int cex[BS + 1];  // exclusive scan of the ranges' chunk counts; cex[n] = chunks in the batch
+    Ent ent[BS];
+    int wsum[2][NW];
+};
+
+// LDS a kernel may hold per block on gfx950 (static + dynamic).
+constexpr size_t kMaxLds = 160 * 1024;
+
+// Stages one batch from per-thread ranges (thread t holds A-nonzero e with B
+// range [bs, bs + len) and A value av; len = 0 for none): the non-empty
+// ranges in thread order with their chunk offsets.  Returns their number;
+// `chunks` gets the batch's chunk count.  Every thread of the block calls it;
+// it ends with a barrier.  The previous walk of `st` must be complete (a
+// barrier) before.
+template <int BS, typename V>
+__device__ __forceinline__ int chunk_stage_ranges(int e, int bs, int len, V av, ChunkStage<BS, V> &st, int &chunks)
+{
+    constexpr int NW = BS / kWave;
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid / kWave;
+    const int f = len > 0 ? 1 : 0;
+    const int nch = (len + kWave - 1) / kWave;
+    const int ic = wave_incl_scan(nch);
+    const int iff = wave_incl_scan(f);
+    if (lane == kWave - 1) {
+        st.wsum[0][wave] = ic;
+        st.wsum[1][wave] = iff;
+    }
+    __syncthreads();
+    int pc = 0, pf = 0, tc = 0, tf = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const int a = st.wsum[0][w], b = st.wsum[1][w];
+        if (w < wave) {
+            pc += a;
+            pf += b;
+        }
+        tc += a;
+        tf += b;
+    }
+    if (f) {
+        const int ix = pf + iff - 1;
+        st.cex[ix] = pc + ic - nch;
+        st.ent[ix] = typename ChunkStage<BS, V>::Ent{bs, len, e};
+        st.av[ix] = av;
+    }
+    if (tid == 0) st.cex[tf] = tc;
+    __syncthreads();
+    chunks = tc;
+    return tf;
+}
+
+template <int BS, typename V, typename Range>
+__device__ __forceinline__ int chunk_stage(int c, int a_end, const int *__restrict__ acol, const V *__restrict__ aval,
+                                           const Range &range, ChunkStage<BS, V> &st, int &chunks)
+{
+    const int e = c + static_cast<int>(threadIdx.x);
+    int bs = 0, len = 0;
+    V av = V(0);
+    if (e < a_end) {
+        range(e, acol[e], bs, len);
+        if (aval) av = aval[e];
+    }
+    return chunk_stage_ranges<BS, V>(e, bs, len, av, st, chunks);
+}
+
+// Walks every product of a staged batch (n ranges, `chunks` chunks) with all
+// waves of the block: wave w takes chunks [w * per, (w + 1) * per), per =
+// ceil(chunks / NW), one 64-lane step per chunk, NP steps gathered at once
+// in two ping-pong buffers (every load issued inside the loop, so the
+// compiler waits for one buffer at a time).  The current range is held in
+// scalar registers and advances by one when a step reaches the next range's
+// first chunk (every staged range has at least one chunk).  Lanes past the
+// range's end get ok = false and the batch's first B index, so every load is
+// issued unconditionally and stays in bounds.  Ends with a barrier.
+template <int BS, bool WITH_A, typename V, typename Op>
+__device__ __forceinline__ void chunk_walk(const ChunkStage<BS, V> &st, int n, int chunks, Op &op)
+{
+    constexpr int NW = BS / kWave;
+    constexpr int NP = 4;
+    static_assert(BS <= kWave * kWave, "two-round owner search");
+    const int lane = lane_id();
+    // wave-uniform in scalar registers, so every branch on chunk indices
+    // below is a scalar branch
+    const int wave = wave_uniform(static_cast<int>(threadIdx.x / kWave));
+    chunks = wave_uniform(chunks);
+    n = wave_uniform(n);
+    const int per = (chunks + NW - 1) / NW;
+    const int j0 = wave * per;
+    const int j1 = min(chunks, j0 + per);
+    if (j0 < j1) {
+        // Owner of chunk j0: largest q < n with cex[q] <= j0 (64-ary search in two rounds).
+        const int g1 = (n + kWave - 1) / kWave;
+        int q = lane * g1;
+        int cnt = __popcll(__ballot(q < n && st.cex[q] <= j0));
+        const int lo = (cnt - 1) * g1;
+        q = lo + lane;
+        cnt = __popcll(__ballot(lane < g1 && q < n && st.cex[q] <= j0));
+        int qa = wave_uniform(lo + cnt - 1);
+        int c_lo = 0, c_hi = 0, r_bs = 0, r_len = 0, r_e = 0;
+        V r_av = V(0);
+        auto load_q = [&]() {
+            c_lo = wave_uniform(st.cex[qa]);
+            c_hi = wave_uniform(st.cex[qa + 1]);
+            const typename ChunkStage<BS, V>::Ent en = st.ent[qa];
+            r_bs = wave_uniform(en.bs);
+            r_len = wave_uniform(en.len);
+            r_e = wave_uniform(en.e);
+            if constexpr (WITH_A) r_av = wave_uniform(st.av[qa]);
+        };
+        load_q();
+        const int safe = wave_uniform(st.ent[0].bs);
+        auto gather = [&](int jb, int (&bi)[NP], V (&aa)[NP], int (&qq)[NP], bool (&ok)[NP]) {
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                const int jj = jb + s;
+                ok[s] = false;
+                bi[s] = safe;
+                aa[s] = V(0);
+                qq[s] = 0;
+                if (jj < j1) {
+                    if (jj == c_hi) {
+                        qa++;
+                        load_q();
+                    }
+                    const int pos = (jj - c_lo) * kWave + lane;
+                    ok[s] = pos < r_len;
+                    bi[s] = ok[s] ? r_bs + pos : safe;
+                    aa[s] = r_av;
+                    qq[s] = r_e;
+                }
+            }
+        };
+        int bbA[NP], qqA[NP], bbB[NP], qqB[NP];
+        V aaA[NP], aaB[NP];
+        bool okA[NP], okB[NP];
+        typename Op::Item itA[NP], itB[NP];
+#pragma unroll
+        for (int s = 0; s < NP; s++) {
+            okB[s] = false;
+            qqB[s] = 0;
+            aaB[s] = V(0);
+            itB[s] = typename Op::Item{};
+        }
+        for (int jb = j0;; jb += 2 * NP) {
+            gather(jb, bbA, aaA, qqA, okA);
+#pragma unroll
+            for (int s = 0; s < NP; s++) itA[s] = op.load(bbA[s]);
+            apply_batch<NP>(op, itB, aaB, qqB, okB);
+            gather(jb + NP, bbB, aaB, qqB, okB);
+#pragma unroll
+            for (int s = 0; s < NP; s++) itB[s] = op.load(bbB[s]);
+            apply_batch<NP>(op, itA, aaA, qqA, okA);
+            if (jb + 2 * NP >= j1) break;
+        }
+        apply_batch<NP>(op, itB, aaB, qqB, okB);
+    }
+    __syncthreads();
+}
+
+// Every product of A row segment [a_beg, a_end) with the whole block, in
+// batches of BS A-nonzeros (chunk_stage + chunk_walk).  The consumer's q
+// argument is the A-nonzero index.  Returns true when the row fit in one
+// batch: `st` then still holds it for another walk over the same ranges
+// (chunk_walk with the returned n and chunks).
+template <int BS, bool WITH_A, typename V, typename Range, typename Op>
+__device__ __forceinline__ bool chunk_for_each_product(int a_beg, int a_end, const int *__restrict__ acol,
+                                                       const V *__restrict__ aval, const Range &range, Op &op,
+                                                       ChunkStage<BS, V> &st, int &n, int &chunks)
+{
+    n = 0;
+    chunks = 0;
+    for (int c = a_beg; c < a_end; c += BS) {
+        n = chunk_stage<BS, V>(c, a_end, acol, aval, range, st, chunks);
+        if (chunks > 0) chunk_walk<BS, WITH_A, V>(st, n, chunks, op);
+    }
+    return a_end - a_beg <= BS;
+}
+
 // First index in [lo, t) of the sorted B row segment whose column is >=
 // bound (t if none): galloping from lo, then a binary search, so the cost
 // grows with the distance from lo rather than with the row length.

@@ -226,6 +226,7 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
     __shared__ Acc sda[kLongBS];
     __shared__ int s_cnt[kLongBS / kWave];
     __shared__ BlockStage<kLongBS, V> s_bst;
+    static_assert(R::lds + sizeof(Acc) * kLongBS + sizeof(BlockStage<kLongBS, V>) + 256 <= kMaxLds, "LDS");
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int lane = lane_id();

@@ -192,7 +192,8 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
     extern __shared__ unsigned bm[];
     __shared__ int s_red[kLongBS / kWave];
     __shared__ int s_slot;
-    __shared__ BlockStage<kLongBS, double> s_bst;
+    __shared__ ChunkStage<kLongBS, double> s_bst;
+    static_assert(sizeof(unsigned) * kBitmapWords + sizeof(ChunkStage<kLongBS, double>) + 256 <= kMaxLds, "LDS");
     const int tid = threadIdx.x;
     // The bitmap is cleared once here and then by each window's count pass.
     for (int i = tid; i < (wcols >> 5); i += kLongBS) bm[i] = 0u;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
             drain_vmem();
             BitmapInsert op{bcol, bm, w0};
             int sn, stot;
-            block_for_each_product<kLongBS, false, double>(a_beg, a_end, acol, static_cast<const double *>(nullptr),
+            chunk_for_each_product<kLongBS, false, double>(a_beg, a_end, acol, static_cast<const double *>(nullptr),
                                                            range, op, s_bst, sn, stot);
             __syncthreads();
             if (save.bm) {

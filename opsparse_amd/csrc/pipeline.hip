@@ -190,9 +190,15 @@ void validate(const ops_csr *A, const ops_csr *B)
 void init_attrs()
 {
     if (g_attrs_done) return;
+    (void)hipGetLastError();
     symbolic_init_attributes();
     numeric_init_attributes();
     backward_init_attributes();
+    // An attribute the device refuses (a kernel asking for more LDS than a
+    // block may hold) is reported here, not by a later launch check.
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        throw HipFail(OPS_HIP_ERROR, std::string("kernel attribute setup: ") + hipGetErrorString(e));
     g_attrs_done = true;
 }
 
