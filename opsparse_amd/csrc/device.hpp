@@ -1212,4 +1212,19 @@ __device__ __forceinline__ T block_sum(T v, T *scratch)
     return s;
 }
 
+template <int BS>
+__device__ __forceinline__ int block_min(int v, int *scratch)
+{
+    const int lane = lane_id();
+    const int wid = threadIdx.x / kWave;
+    v = unit_min<kWave>(v);
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    int m = scratch[0];
+#pragma unroll
+    for (int w = 1; w < BS / kWave; w++) m = min(m, scratch[w]);
+    __syncthreads();
+    return m;
+}
+
 }  // namespace opsamd

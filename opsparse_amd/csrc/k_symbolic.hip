@@ -148,12 +148,15 @@ constexpr int kLongBS = 1024;
 constexpr int kBitmapWords = kSymLongWindow / 32;  // 128 KB
 
 // B range of an A-nonzero restricted to columns [w0, wend).  For a single
-// window covering all columns this is the whole row.
+// window covering all columns this is the whole row.  Otherwise *next
+// (thread-local) takes the minimum over the calls of the first column at or
+// past wend, so the caller can jump over windows no product falls in.
 struct WindowRange {
     const int *__restrict__ brpt;
     const int *__restrict__ bcol;
     int w0, wend;
     bool whole;
+    int *next;
     __device__ __forceinline__ void operator()(int /*e*/, int k, int &bs, int &len) const
     {
         const int s = brpt[k], t = brpt[k + 1];
@@ -181,6 +184,7 @@ struct WindowRange {
         }
         bs = first;
         len = lo - first;
+        if (lo < t) *next = min(*next, bcol[lo]);
     }
 };
 
@@ -202,10 +206,11 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
         const int r = rows ? rows[ridx] : ridx;
         const int a_beg = arpt[r], a_end = arpt[r + 1];
         int cnt = 0;
-        for (int w0 = 0; w0 < ncols; w0 += wcols) {
-            const int wend = min(ncols, w0 + wcols);
+        for (int w0 = 0; w0 < ncols;) {
+            const int wend = static_cast<int>(min(static_cast<long long>(ncols), static_cast<long long>(w0) + wcols));
             const int nwords = (wend - w0 + 31) >> 5;
-            const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols};
+            int next = INT_MAX;
+            const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols, &next};
             drain_vmem();
             BitmapInsert op{bcol, bm, w0};
             int sn, stot;
@@ -246,6 +251,12 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
                 }
             }
             __syncthreads();
+            if (wend == ncols) break;
+            // Next window holding a product of this row: windows with none
+            // cost nothing, so the row's cost follows its products, not B.N.
+            next = block_min<kLongBS>(next, s_red);
+            if (next == INT_MAX) break;
+            w0 = (next / wcols) * wcols;
         }
         const int tot = save.bm ? cnt : block_sum<kLongBS>(cnt, s_red);
         if (tid == 0) row_nnz[r] = tot;

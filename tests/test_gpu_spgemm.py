@@ -152,6 +152,40 @@ def test_symbolic_multi_window(gpu):
     run(A, B)
 
 
+def test_symbolic_long_rows_skip_empty_windows(gpu):
+    """B with 2^24 columns (16 bitmap windows) whose columns fall in a few
+    windows: the long-row symbolic kernel visits only windows holding a
+    product, so its time stays flat as B.N grows from 2^21 to 2^24 (it
+    would grow about 8x if every window were walked)."""
+    from opsparse_amd._lib import OpsTimings
+    rng = np.random.default_rng(23)
+    K, M = 400, 96
+    A = random_csr(M, K, rng.integers(150, 250, M), rng)
+    # B's columns in windows 0 and 1 of 2^20 columns, plus, in the spread
+    # case, a few in windows 9 and 15 (hit by a third of B's rows)
+    base = random_csr(K, 1 << 21, rng.integers(150, 300, K), rng)
+    rows = [base.col[base.rpt[k]:base.rpt[k + 1]] for k in range(K)]
+    far = [np.unique(np.r_[r, (9 << 20) + r[:3], (15 << 20) + 5 + r[:2]]) if k % 3 == 0 else r
+           for k, r in enumerate(rows)]
+    cases = {"narrow": ((1 << 21), rows), "wide": ((1 << 24), rows), "spread": ((1 << 24), far)}
+    ms = {}
+    for name, (N, rr) in cases.items():
+        B = csr_from_rows(K, N, rr, rng)
+        dA, dB = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+        best = None
+        for _ in range(3):
+            t = OpsTimings()
+            t.kernel_events = 1
+            spgemm(dA, dB, timings=t).free()
+            best = t.sym_ms[7] if best is None else min(best, t.sym_ms[7])
+        assert t.sym_rows[7] == M, list(t.sym_rows)
+        ms[name] = best
+        run(A, B)
+    log(f"long-row symbolic ms by case: {ms}")
+    assert ms["wide"] <= 2.0 * ms["narrow"] + 0.05, ms
+    assert ms["spread"] <= 3.0 * ms["narrow"] + 0.05, ms
+
+
 def test_symbolic_bin6_hash_or_bitmap(gpu):
     """Rows of 4097..16384 products: counted by the bin-6 hash kernel when B
     has more columns than one long-row bitmap window (2^20), by the bitmap
