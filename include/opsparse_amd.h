@@ -241,6 +241,42 @@ int ops_plan_numeric(ops_plan *plan, ops_dtype dt, const ops_csr *A, const ops_c
 int ops_plan_numeric_graph(ops_plan *plan, ops_dtype dt, const ops_csr *A, const ops_csr *B, int32_t *ccol,
                            void *cval, void *stream);
 
+/* ---- Whole calls replayed from a hipGraph (no counterpart in the reference)
+ *
+ * An executor runs the whole product - setup, binning, symbolic, scan and
+ * numeric - with bin sizes, bin offsets and nnz(C) kept on the device: no
+ * host synchronisation, every per-bin kernel launched for every bin (each
+ * reads its bin's size on the device and loops over its rows).  The
+ * sequence is captured once into a hipGraph (use_graph = 1; 0 issues it
+ * again at every launch) and each ops_exec_launch replays it.  The
+ * operands' descriptors (pointers, M, N, nnz) are fixed at creation; their
+ * row offsets, columns and values may change between launches (the entry
+ * counts staying the same).  C lives in the executor: 64-bit row offsets
+ * and room for `capacity` entries (capacity <= 0: exactly nnz(C) of the
+ * operands at creation, measured by one untimed run).  The workspace lends
+ * its streams and tuning (taken at creation) and must outlive the executor.
+ * Long rows take the cursor path (no kept column bitmaps). */
+typedef struct ops_exec ops_exec;
+int ops_exec_create(ops_workspace *ws, ops_dtype dt, const ops_csr *A, const ops_csr *B, int64_t capacity,
+                    int use_graph, ops_exec **ex);
+/* Batched: C_i = A_i * B_i for `count` independent instances (host arrays
+ * of device descriptors, as for ops_batch_pack).  Each launch packs the A_i
+ * and the B_i block-diagonally (kernels over instance tables built here),
+ * multiplies, and shifts every C_i back to its own columns: C_i is rows
+ * [row_base[i], row_base[i+1]) of the executor's C, with B_i's column
+ * indices.  row_base and col_base (count + 1 host entries, may be NULL)
+ * receive the A_i row bases and the B_i column bases. */
+int ops_exec_create_batch(ops_workspace *ws, ops_dtype dt, int32_t count, const ops_csr *A, const ops_csr *B,
+                          int64_t capacity, int use_graph, int64_t *row_base, int64_t *col_base, ops_exec **ex);
+/* Enqueues one whole call on `stream` (returns at once). */
+int ops_exec_launch(ops_exec *ex, void *stream);
+/* Waits for `stream`, then describes the last launch's C (arrays owned by
+ * the executor, valid until the next launch or ops_exec_destroy) and its
+ * nnz.  Returns OPS_INVALID when nnz(C) exceeded the capacity: C->nnz and
+ * the row offsets are then set, no entry was written. */
+int ops_exec_result(ops_exec *ex, ops_csr64 *C, int64_t *nnz_c, void *stream);
+int ops_exec_destroy(ops_exec *ex);
+
 /* ---- Tuning: bin boundaries and the short-row hash family ---------------- */
 
 /* The reference hard-codes its bin bounds (kernels/binning.cuh:21, 51, 105,

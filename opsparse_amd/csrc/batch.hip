@@ -1,4 +1,5 @@
-// Block-diagonal packing of independent CSR matrices (ops_batch_pack).
+// Block-diagonal packing of independent CSR matrices (ops_batch_pack, and
+// the packing step of a batched executor, ops_exec_create_batch).
 //
 // One block per non-empty instance copies the instance's rows into its
 // slice of the packed arrays, rebasing row offsets and shifting columns by
@@ -17,8 +18,11 @@
 #include <cstring>
 #include <vector>
 
+#include "batch.hpp"
 #include "opsparse_amd.h"
 #include "outmem.hpp"
+
+namespace opsamd {
 
 namespace {
 
@@ -63,15 +67,39 @@ __global__ void k_set_last(int *prpt, int M, int nnz)
     if (threadIdx.x == 0) prpt[M] = nnz;
 }
 
+// Columns of C_i back to instance i's own coordinates: block i takes the
+// entries of rows [row_base[i], row_base[i+1]) and subtracts col_base[i].
+__global__ __launch_bounds__(256) void k_unshift(const long long *__restrict__ crpt, int *__restrict__ ccol,
+                                                 const int *__restrict__ row_base, const int *__restrict__ col_base)
+{
+    const int i = blockIdx.x;
+    const int cb = col_base[i];
+    if (cb == 0) return;
+    const long long e0 = crpt[row_base[i]], e1 = crpt[row_base[i + 1]];
+    for (long long e = e0 + threadIdx.x; e < e1; e += blockDim.x) ccol[e] -= cb;
+}
+
+int status_of(hipError_t e)
+{
+    (void)hipGetLastError();
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? OPS_OOM : OPS_HIP_ERROR;
+}
+
 }  // namespace
 
-extern "C" int ops_batch_pack(int32_t count, const ops_csr *instances, ops_dtype dt, ops_csr *packed,
-                              int64_t *row_base, int64_t *col_base, void *stream)
+void PackedSide::release()
 {
-    if (!packed || count < 0 || (count > 0 && !instances)) return OPS_INVALID;
-    if (dt != OPS_F64 && dt != OPS_F32) return OPS_INVALID;
-    *packed = ops_csr{};
-    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (inst) (void)hipFree(inst);
+    inst = nullptr;
+    nlive = 0;
+}
+
+int pack_side_prepare(int count, const ops_csr *instances, PackedSide &side, int64_t *row_base, int64_t *col_base,
+                      hipStream_t s)
+{
+    side.release();
+    side.rows = side.cols = side.nnz = 0;
+    if (count < 0 || (count > 0 && !instances)) return OPS_INVALID;
     // Host checks and the instances that own rows.
     std::vector<int> live;
     live.reserve(static_cast<size_t>(count));
@@ -83,39 +111,33 @@ extern "C" int ops_batch_pack(int32_t count, const ops_csr *instances, ops_dtype
     }
     const int nlive = static_cast<int>(live.size());
     std::vector<InstDesc> h(static_cast<size_t>(nlive));
+    std::vector<int> ends(2 * static_cast<size_t>(nlive));
     InstDesc *d_inst = nullptr;
     int *d_ends = nullptr;
-    auto cleanup = [&]() {
+    auto fail = [&](int st) {
         if (d_inst) (void)hipFree(d_inst);
         if (d_ends) (void)hipFree(d_ends);
-        d_inst = nullptr;
-        d_ends = nullptr;
-    };
-    auto fail = [&](hipError_t e) {
-        const int st = (e == hipErrorOutOfMemory) ? OPS_OOM : OPS_HIP_ERROR;
-        (void)hipGetLastError();
-        cleanup();
-        ops_csr_free(packed);
         return st;
     };
     hipError_t e;
-    std::vector<int> ends(2 * static_cast<size_t>(nlive));
     if (nlive > 0) {
         for (int j = 0; j < nlive; j++) {
             const ops_csr &c = instances[live[j]];
             h[j] = InstDesc{c.rpt, c.col, c.val, c.M, 0, 0, 0};
         }
-        if ((e = hipMalloc(&d_inst, sizeof(InstDesc) * h.size())) != hipSuccess) return fail(e);
-        if ((e = hipMalloc(&d_ends, sizeof(int) * ends.size())) != hipSuccess) return fail(e);
+        if ((e = hipMalloc(&d_inst, sizeof(InstDesc) * h.size())) != hipSuccess) return fail(status_of(e));
+        if ((e = hipMalloc(&d_ends, sizeof(int) * ends.size())) != hipSuccess) return fail(status_of(e));
         if ((e = hipMemcpyAsync(d_inst, h.data(), sizeof(InstDesc) * h.size(), hipMemcpyHostToDevice, s)) !=
             hipSuccess)
-            return fail(e);
+            return fail(status_of(e));
         k_row_ends<<<(nlive + 255) / 256, 256, 0, s>>>(d_inst, nlive, d_ends);
-        if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(status_of(e));
         if ((e = hipMemcpyAsync(ends.data(), d_ends, sizeof(int) * ends.size(), hipMemcpyDeviceToHost, s)) !=
             hipSuccess)
-            return fail(e);
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e);
+            return fail(status_of(e));
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(status_of(e));
+        (void)hipFree(d_ends);
+        d_ends = nullptr;
     }
     long long rows = 0, cols = 0, nnz = 0;
     for (int i = 0, j = 0; i < count; i++) {
@@ -124,10 +146,7 @@ extern "C" int ops_batch_pack(int32_t count, const ops_csr *instances, ops_dtype
         if (col_base) col_base[i] = cols;
         if (c.M > 0) {
             const long long n = static_cast<long long>(ends[2 * j + 1]) - ends[2 * j];
-            if (n < 0 || n != c.nnz) {
-                cleanup();
-                return OPS_INVALID;  // rpt[M] - rpt[0] must be the descriptor's nnz
-            }
+            if (n < 0 || n != c.nnz) return fail(OPS_INVALID);  // rpt[M] - rpt[0] must be the descriptor's nnz
             h[j].row_base = static_cast<int>(rows);
             h[j].nnz_base = nnz;
             h[j].col_base = static_cast<int>(cols);
@@ -136,38 +155,75 @@ extern "C" int ops_batch_pack(int32_t count, const ops_csr *instances, ops_dtype
         }
         rows += c.M;
         cols += c.N;
-        if (rows > INT32_MAX || cols > INT32_MAX || nnz > INT32_MAX) {
-            cleanup();
-            return OPS_INT32_OVERFLOW;
-        }
+        if (rows > INT32_MAX || cols > INT32_MAX || nnz > INT32_MAX) return fail(OPS_INT32_OVERFLOW);
     }
     if (row_base) row_base[count] = rows;
     if (col_base) col_base[count] = cols;
-    const size_t vsz = dt == OPS_F64 ? sizeof(double) : sizeof(float);
-    if ((e = opsamd::out_malloc(reinterpret_cast<void **>(&packed->rpt),
-                                sizeof(int) * (static_cast<size_t>(rows) + 1))) != hipSuccess)
-        return fail(e);
-    if ((e = opsamd::out_malloc(reinterpret_cast<void **>(&packed->col),
-                                sizeof(int) * static_cast<size_t>(nnz > 0 ? nnz : 1))) != hipSuccess)
-        return fail(e);
-    if ((e = opsamd::out_malloc(&packed->val, vsz * static_cast<size_t>(nnz > 0 ? nnz : 1))) != hipSuccess)
-        return fail(e);
-    packed->M = static_cast<int32_t>(rows);
-    packed->N = static_cast<int32_t>(cols);
-    packed->nnz = static_cast<int32_t>(nnz);
     if (nlive > 0) {
         if ((e = hipMemcpyAsync(d_inst, h.data(), sizeof(InstDesc) * h.size(), hipMemcpyHostToDevice, s)) !=
             hipSuccess)
-            return fail(e);
-        if (dt == OPS_F64)
-            k_pack<double><<<nlive, 256, 0, s>>>(d_inst, packed->rpt, packed->col, static_cast<double *>(packed->val));
-        else
-            k_pack<float><<<nlive, 256, 0, s>>>(d_inst, packed->rpt, packed->col, static_cast<float *>(packed->val));
-        if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+            return fail(status_of(e));
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(status_of(e));
     }
-    k_set_last<<<1, 64, 0, s>>>(packed->rpt, static_cast<int>(rows), static_cast<int>(nnz));
+    side.inst = d_inst;
+    side.nlive = nlive;
+    side.rows = static_cast<int>(rows);
+    side.cols = static_cast<int>(cols);
+    side.nnz = static_cast<int>(nnz);
+    return OPS_OK;
+}
+
+void pack_side_launch(const PackedSide &side, bool f64, int *prpt, int *pcol, void *pval, hipStream_t s)
+{
+    const InstDesc *inst = static_cast<const InstDesc *>(side.inst);
+    if (side.nlive > 0) {
+        if (f64)
+            k_pack<double><<<side.nlive, 256, 0, s>>>(inst, prpt, pcol, static_cast<double *>(pval));
+        else
+            k_pack<float><<<side.nlive, 256, 0, s>>>(inst, prpt, pcol, static_cast<float *>(pval));
+    }
+    k_set_last<<<1, 64, 0, s>>>(prpt, side.rows, side.nnz);
+}
+
+void launch_unshift(const long long *crpt, int *ccol, const int *row_base, const int *col_base, int count,
+                    hipStream_t s)
+{
+    if (count > 0) k_unshift<<<count, 256, 0, s>>>(crpt, ccol, row_base, col_base);
+}
+
+}  // namespace opsamd
+
+extern "C" int ops_batch_pack(int32_t count, const ops_csr *instances, ops_dtype dt, ops_csr *packed,
+                              int64_t *row_base, int64_t *col_base, void *stream)
+{
+    if (!packed || count < 0 || (count > 0 && !instances)) return OPS_INVALID;
+    if (dt != OPS_F64 && dt != OPS_F32) return OPS_INVALID;
+    *packed = ops_csr{};
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    opsamd::PackedSide side;
+    int st = opsamd::pack_side_prepare(count, instances, side, row_base, col_base, s);
+    if (st != OPS_OK) return st;
+    auto fail = [&](hipError_t e) {
+        st = opsamd::status_of(e);
+        side.release();
+        ops_csr_free(packed);
+        return st;
+    };
+    hipError_t e;
+    const size_t vsz = dt == OPS_F64 ? sizeof(double) : sizeof(float);
+    const size_t nz = static_cast<size_t>(side.nnz > 0 ? side.nnz : 1);
+    if ((e = opsamd::out_malloc(reinterpret_cast<void **>(&packed->rpt),
+                                sizeof(int) * (static_cast<size_t>(side.rows) + 1))) != hipSuccess)
+        return fail(e);
+    if ((e = opsamd::out_malloc(reinterpret_cast<void **>(&packed->col), sizeof(int) * nz)) != hipSuccess)
+        return fail(e);
+    if ((e = opsamd::out_malloc(&packed->val, vsz * nz)) != hipSuccess) return fail(e);
+    packed->M = side.rows;
+    packed->N = side.cols;
+    packed->nnz = side.nnz;
+    opsamd::pack_side_launch(side, dt == OPS_F64, packed->rpt, packed->col, packed->val, s);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e);
-    cleanup();
+    side.release();
     return OPS_OK;
 }

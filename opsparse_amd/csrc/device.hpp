@@ -214,8 +214,10 @@ __device__ __forceinline__ void apply_batch(Op &op, const typename Op::Item (&it
 }
 
 // Grouped walk of one loaded chunk: GS lanes per A-nonzero, each group
-// strides over its B row four entries per lane at a time.  No per-product cross-lane
-// search; used when the chunk's B rows have similar lengths.
+// strides over its B row NPG entries per lane at a time (8 for groups of 4
+// lanes, so a B row of up to 32 entries takes one step; 4 otherwise).  No
+// per-product cross-lane search; used when the chunk's B rows have similar
+// lengths.
 template <int U, int GS, bool WITH_A, typename V, bool SLICE, typename Op>
 __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op &op, int part, int nparts)
 {
@@ -233,7 +235,7 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
     // rows; group distribution suits many A-nonzeros with short B rows.
     // Each lane takes NPG entries of its group's B row per step (their loads
     // are issued together).
-    constexpr int NPG = 4;
+    constexpr int NPG = GS == 4 ? 8 : 4;
     const int t0 = SLICE ? sub + part * NPG * GS : sub;
     const int tstep = SLICE ? nparts * NPG * GS : NPG * GS;
     const int qstart = SLICE ? 0 : part * PER;
@@ -248,8 +250,13 @@ __device__ __forceinline__ void chunk_grouped(int cnt, int bs, int len, V av, Op
         const int ql = q < cnt ? lq : 0;
         V qa = V(0);
         if constexpr (WITH_A) qa = bperm(av, src);
-        const V aa[NPG] = {qa, qa, qa, qa};
-        const int qq[NPG] = {q, q, q, q};
+        V aa[NPG];
+        int qq[NPG];
+#pragma unroll
+        for (int n = 0; n < NPG; n++) {
+            aa[n] = qa;
+            qq[n] = q;
+        }
         for (int t = t0; t < ql; t += tstep) {
             typename Op::Item it[NPG];
             bool ok[NPG];
@@ -335,9 +342,11 @@ __device__ __forceinline__ bool walk_chunk(int cnt, int bs, int len, V av, Op &o
     const int mean = (total + cnt - 1) / cnt;
     if (mx <= 2 * mean) {
         // Groups of about half the mean length: several products per lane per
-        // round amortise the per-round cross-lane reads.
+        // round amortise the per-round cross-lane reads.  Below 32 lanes'
+        // worth, groups of 4 lanes taking 8 entries each: fewer rounds than
+        // groups of 8 taking 4.
         const int half = mean >> 1;
-        const int gs = half >= 64 ? 64 : (half >= 32 ? 32 : (half >= 16 ? 16 : (half >= 8 ? 8 : 4)));
+        const int gs = half >= 64 ? 64 : (half >= 32 ? 32 : (half >= 16 ? 16 : 4));
         if (gs <= 4) {
             chunk_grouped<U, 4, WITH_A, V, SLICE>(cnt, bs, len, av, op, part, nparts);
         } else if (gs == 8 || U == 8) {
@@ -1026,6 +1035,19 @@ __device__ __forceinline__ int hash_insert(int *tab, int key, bool &fresh)
         }
         h = (h + 1) & (T - 1);
     }
+}
+
+// Resolves a RowList on the device: returns the row count and sets `rows`
+// to the list (nullptr: rows 0..n-1).  The values are uniform across the
+// grid (scalar loads).
+__device__ __forceinline__ int row_list(const RowList &rl, const int *&rows)
+{
+    rows = rl.rows;
+    if (!rl.dsize) return rl.n;
+    int off = 0;
+    for (int b = 0; b < rl.bin; b++) off += rl.dsize[b];
+    rows += off;
+    return rl.dsize[rl.bin];
 }
 
 // Blocks are dispatched to the 8 XCDs round-robin (block b runs on XCD

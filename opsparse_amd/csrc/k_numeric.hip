@@ -54,9 +54,8 @@ template <int G, int T, int NMAX, int BS, int H, typename V>
 __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                   const V *__restrict__ aval, const int *__restrict__ brpt,
                                                   const int *__restrict__ bcol, const V *__restrict__ bval,
-                                                  const int *__restrict__ rows, int nrows,
-                                                  const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                  V *__restrict__ cval, bool pack)
+                                                  const RowList rl, const long long *__restrict__ crpt,
+                                                  int *__restrict__ ccol, V *__restrict__ cval, bool pack)
 {
     constexpr int U = G;
     constexpr int RPB = BS / G;
@@ -70,62 +69,68 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     const int tid = threadIdx.x;
     const int slot = tid / G;
     const int gl = tid % G;
-    for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(keys)[i] = make_int4(-1, -1, -1, -1);
-    for (int i = tid; i < RPB * T * static_cast<int>(sizeof(Acc)) / 16; i += BS)
-        reinterpret_cast<int4 *>(vals)[i] = make_int4(0, 0, 0, 0);
-    __syncthreads();
+    const int *rows;
+    const int nrows = row_list(rl, rows);
+    const int ntiles = (nrows + RPB - 1) / RPB;
+    for (int tile = xcd_tile(); tile < ntiles; tile += gridDim.x) {
+        for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(keys)[i] = make_int4(-1, -1, -1, -1);
+        for (int i = tid; i < RPB * T * static_cast<int>(sizeof(Acc)) / 16; i += BS)
+            reinterpret_cast<int4 *>(vals)[i] = make_int4(0, 0, 0, 0);
+        __syncthreads();
 
-    const int ridx = xcd_tile() * RPB + slot;
-    int r = 0;
-    int *kt = keys + slot * T;
-    Acc *vt = vals + slot * T;
-    if (ridx < nrows) {
-        r = rows ? rows[ridx] : ridx;
-        const FullRow range{brpt};
-        NumInsert<T, H, V> op{bcol, bval, kt, vt};
-        for_each_product<U, true, V>(arpt[r], arpt[r + 1], 0, 1, acol, aval, range, op);
-    }
-    __syncthreads();
-    if (ridx >= nrows) return;
-
-    // Compact the row's keys (slot order) into its key list; the table stays
-    // intact and supplies the values after the keys are sorted.  With pack
-    // (every column below 2^(31 - log2 T)) a key carries its slot in its low
-    // bits, so the sorted list gives each value's slot without a lookup.
-    constexpr int LOGT = __builtin_ctz(T);
-    int *ck = ckeys + slot * NMAX;
-    int n = 0;
-    const unsigned long long below = gl ? ((1ull << gl) - 1ull) : 0ull;
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-        const int key = kt[s * U + gl];
-        const bool valid = key != -1;
-        const unsigned long long m = unit_ballot<U>(valid);
-        const int pos = n + __popcll(m & below);
-        if (valid && pos < NMAX) ck[pos] = pack ? ((key << LOGT) | (s * U + gl)) : key;
-        n += __popcll(m);
-    }
-    n = min(n, NMAX);
-    wave_lds_sync();
-
-    int sk[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int j = k * U + gl;
-        sk[k] = j < n ? ck[j] : INT_MAX;
-    }
-    unit_bitonic_sort_keys<U, K>(sk);
-    const long long off = crpt[r];
-    const int n_out = min(n, static_cast<int>(crpt[r + 1] - off));  // never write past the row
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int j = k * U + gl;
-        if (j < n_out) {
-            const int key = pack ? (sk[k] >> LOGT) : sk[k];
-            const int h = pack ? (sk[k] & (T - 1)) : hash_find<T, H>(kt, sk[k]);
-            ccol[off + j] = key;
-            cval[off + j] = h >= 0 ? static_cast<V>(vt[h]) : V(0);
+        const int ridx = tile * RPB + slot;
+        int r = 0;
+        int *kt = keys + slot * T;
+        Acc *vt = vals + slot * T;
+        if (ridx < nrows) {
+            r = rows ? rows[ridx] : ridx;
+            const FullRow range{brpt};
+            NumInsert<T, H, V> op{bcol, bval, kt, vt};
+            for_each_product<U, true, V>(arpt[r], arpt[r + 1], 0, 1, acol, aval, range, op);
         }
+        __syncthreads();
+        if (ridx < nrows) {
+            // Compact the row's keys (slot order) into its key list; the table stays
+            // intact and supplies the values after the keys are sorted.  With pack
+            // (every column below 2^(31 - log2 T)) a key carries its slot in its low
+            // bits, so the sorted list gives each value's slot without a lookup.
+            constexpr int LOGT = __builtin_ctz(T);
+            int *ck = ckeys + slot * NMAX;
+            int n = 0;
+            const unsigned long long below = gl ? ((1ull << gl) - 1ull) : 0ull;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const int key = kt[s * U + gl];
+                const bool valid = key != -1;
+                const unsigned long long m = unit_ballot<U>(valid);
+                const int pos = n + __popcll(m & below);
+                if (valid && pos < NMAX) ck[pos] = pack ? ((key << LOGT) | (s * U + gl)) : key;
+                n += __popcll(m);
+            }
+            n = min(n, NMAX);
+            wave_lds_sync();
+
+            int sk[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = k * U + gl;
+                sk[k] = j < n ? ck[j] : INT_MAX;
+            }
+            unit_bitonic_sort_keys<U, K>(sk);
+            const long long off = crpt[r];
+            const int n_out = min(n, static_cast<int>(crpt[r + 1] - off));  // never write past the row
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = k * U + gl;
+                if (j < n_out) {
+                    const int key = pack ? (sk[k] >> LOGT) : sk[k];
+                    const int h = pack ? (sk[k] & (T - 1)) : hash_find<T, H>(kt, sk[k]);
+                    ccol[off + j] = key;
+                    cval[off + j] = h >= 0 ? static_cast<V>(vt[h]) : V(0);
+                }
+            }
+        }
+        __syncthreads();  // the next tile reinitialises the tables
     }
 }
 
@@ -192,14 +197,12 @@ template <int T, int BS, typename V>
 __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                   const V *__restrict__ aval, const int *__restrict__ brpt,
                                                   const int *__restrict__ bcol, const V *__restrict__ bval,
-                                                  const int *__restrict__ rows, int nrows,
-                                                  const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                  V *__restrict__ cval, const int *__restrict__ dcount)
+                                                  const RowList rl, const long long *__restrict__ crpt,
+                                                  int *__restrict__ ccol, V *__restrict__ cval)
 {
     extern __shared__ unsigned long long smem_u64[];
-    // dcount (optional): the number of rows, read on the device (rows that
-    // a previous kernel of the same stream appended to a list).
-    if (dcount) nrows = *dcount;
+    const int *rows;
+    const int nrows = row_list(rl, rows);
     // Hash path: T values | T keys | T/2 sorted keys.
     Acc *vt = reinterpret_cast<Acc *>(smem_u64);
     int *kt = reinterpret_cast<int *>(vt + T);
@@ -378,9 +381,9 @@ template <int NMAX, int W, int WPB, typename V>
 __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                            const V *__restrict__ aval, const int *__restrict__ brpt,
                                                            const int *__restrict__ bcol, const V *__restrict__ bval,
-                                                           const int *__restrict__ rows, int nrows,
-                                                           const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                           V *__restrict__ cval, int *__restrict__ spill)
+                                                           const RowList rl, const long long *__restrict__ crpt,
+                                                           int *__restrict__ ccol, V *__restrict__ cval,
+                                                           int *__restrict__ spill)
 {
     constexpr int WPL = W / kWave;  // bitmap words per lane in the rank scan
     static_assert(W % kWave == 0 && NMAX * sizeof(Acc) >= NMAX * sizeof(int), "layout");
@@ -392,8 +395,9 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     // start a row (row id, A row, B row offsets, first and last B columns:
     // four dependent steps) are issued for both rows together, so the second
     // row's chain overlaps the first's; the rows are then processed in turn.
-    const int ridx0 = (xcd_tile() * WPB + wave) * 2;
-    if (ridx0 >= nrows) return;  // whole wave: nothing below synchronises the block
+    const int *rows;
+    const int nrows = row_list(rl, rows);
+    const int ntiles = (nrows + 2 * WPB - 1) / (2 * WPB);
     struct Start {
         int r, a_beg, a_end, cbs, clen, mn, mx;
         V cav;
@@ -498,14 +502,19 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
         for (int i = lane; i < n; i += kWave) ccol[off + i] = scol[i];
         wave_lds_sync();  // the next row rewrites this wave's LDS
     };
-    const bool two = ridx0 + 1 < nrows;
-    Start s0, s1;
-    row_start(ridx0, s0);
-    row_start(two ? ridx0 + 1 : ridx0, s1);
-    row_span_rest(s0);
-    row_span_rest(s1);
-    process(s0);
-    if (two) process(s1);
+    // Nothing below synchronises the block: each wave loops on its own.
+    for (int tile = xcd_tile(); tile < ntiles; tile += gridDim.x) {
+        const int ridx0 = (tile * WPB + wave) * 2;
+        if (ridx0 >= nrows) break;  // later tiles start further on
+        const bool two = ridx0 + 1 < nrows;
+        Start s0, s1;
+        row_start(ridx0, s0);
+        row_start(two ? ridx0 + 1 : ridx0, s1);
+        row_span_rest(s0);
+        row_span_rest(s1);
+        process(s0);
+        if (two) process(s1);
+    }
 }
 
 __global__ void k_zero_count(int *p)
@@ -586,9 +595,8 @@ template <typename V>
 __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const V *__restrict__ aval, const int *__restrict__ brpt,
                                                       const int *__restrict__ bcol, const V *__restrict__ bval,
-                                                      const int *__restrict__ rows, int nrows, int ncols,
-                                                      const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                      V *__restrict__ cval, LongScratch ls)
+                                                      const RowList rl, int ncols, const long long *__restrict__ crpt,
+                                                      int *__restrict__ ccol, V *__restrict__ cval, LongScratch ls)
 {
     using R = RankWin<V>;
     constexpr int NU = kLongBS / kWave;
@@ -614,6 +622,8 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     auto cz = [&](int i) { return cursor + (2 + (i & 1)) * span; };
     const int cstride = ls.nwin + 1;
 
+    const int *rows;
+    const int nrows = row_list(rl, rows);
     for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
     __syncthreads();
 
@@ -824,21 +834,21 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
 }
 
 template <int G, int T, int NMAX, int BS, typename V>
-void launch_small(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt, int *ccol,
-                  V *cval, int hash, hipStream_t s)
+void launch_small(const CsrIn &A, const CsrIn &B, const RowList &rl, const long long *crpt, int *ccol, V *cval,
+                  int hash, hipStream_t s)
 {
     constexpr int RPB = BS / G;
-    const int grid = xcd_grid((nrows + RPB - 1) / RPB);
+    const int grid = tile_grid(rl, RPB, kResidentThreads / BS);
     // keys with their slot packed in the low log2(T) bits stay positive
     const bool pack = static_cast<long long>(B.N) <= (1ll << (31 - __builtin_ctz(T)));
     if (hash == kHashLin)
         k_num_small<G, T, NMAX, BS, kHashLin, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val),
-                                                                     B.rpt, B.col, static_cast<const V *>(B.val), rows,
-                                                                     nrows, crpt, ccol, cval, pack);
+                                                                     B.rpt, B.col, static_cast<const V *>(B.val), rl,
+                                                                     crpt, ccol, cval, pack);
     else
         k_num_small<G, T, NMAX, BS, kHashFib, V><<<grid, BS, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val),
-                                                                     B.rpt, B.col, static_cast<const V *>(B.val), rows,
-                                                                     nrows, crpt, ccol, cval, pack);
+                                                                     B.rpt, B.col, static_cast<const V *>(B.val), rl,
+                                                                     crpt, ccol, cval, pack);
 }
 
 template <int T, int BS, typename V>
@@ -848,12 +858,11 @@ constexpr size_t block_lds()
 }
 
 template <int T, int BS, typename V>
-void launch_block(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt, int *ccol,
-                  V *cval, hipStream_t s, const int *dcount = nullptr, int grid = 0)
+void launch_block(const CsrIn &A, const CsrIn &B, const RowList &rl, const long long *crpt, int *ccol, V *cval,
+                  hipStream_t s, int max_blocks)
 {
-    k_num_block<T, BS, V><<<grid > 0 ? grid : nrows, BS, block_lds<T, BS, V>(), s>>>(
-        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows, crpt, ccol,
-        cval, dcount);
+    k_num_block<T, BS, V><<<row_grid(rl, max_blocks), BS, block_lds<T, BS, V>(), s>>>(
+        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, crpt, ccol, cval);
 }
 
 template <typename V>
@@ -876,43 +885,48 @@ void numeric_init_attributes()
 }
 
 template <typename V>
-void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s)
+void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, const long long *crpt, int *ccol,
+                        V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s)
 {
-    if (nrows <= 0) return;
+    if (rl.n <= 0) return;
     switch (bin) {
-        case 0: launch_small<16, 32, 16, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
-        case 1: launch_small<32, 64, 32, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
-        case 2: launch_small<64, 128, 64, 256, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
-        // Bins 3-4: one row (one wave) per block, which measured 2-3% faster
-        // than four rows per block on the stencil and the batch.
-        case 3: launch_small<64, 256, 128, 64, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
-        case 4: launch_small<64, 512, 256, 64, V>(A, B, rows, nrows, crpt, ccol, cval, hash, s); break;
+        case 0: launch_small<16, 32, 16, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
+        case 1: launch_small<32, 64, 32, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
+        case 2: launch_small<64, 128, 64, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
+        // Bin 3: two rows per wave (32 lanes each), 4% faster than one row
+        // per wave on the 27-point stencil (the kernel is VALU-bound there,
+        // and the two rows share its per-step instructions).  Bin 4: one row
+        // (one wave) per block, 2-3% faster than four rows per block.
+        case 3: launch_small<32, 256, 128, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
+        case 4: launch_small<64, 512, 256, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
         case kSpillBin:
             if (spill) {
                 // Wave per row; rows with a wide column span go to the block kernel.
                 constexpr int WPB = 4;
                 k_zero_count<<<1, kWave, 0, s>>>(spill);
-                k_num_wrank<1024, 512, WPB, V><<<xcd_grid((nrows + 2 * WPB - 1) / (2 * WPB)), WPB * kWave, 0, s>>>(
-                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows,
-                    nrows, crpt, ccol, cval, spill);
-                launch_block<2048, 256, V>(A, B, spill + 1, nrows, crpt, ccol, cval, s, spill, std::min(nrows, 4096));
+                k_num_wrank<1024, 512, WPB, V><<<tile_grid(rl, 2 * WPB, kResidentThreads / (WPB * kWave)),
+                                                 WPB * kWave, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val),
+                                                                      B.rpt, B.col, static_cast<const V *>(B.val), rl,
+                                                                      crpt, ccol, cval, spill);
+                // the spilled rows: spill[1 .. spill[0]], counted on the device
+                const RowList sl{spill + 1, rl.n, spill, 0};
+                launch_block<2048, 256, V>(A, B, sl, crpt, ccol, cval, s, 4096);
             } else {
-                launch_block<2048, 256, V>(A, B, rows, nrows, crpt, ccol, cval, s);
+                launch_block<2048, 256, V>(A, B, rl, crpt, ccol, cval, s, 5 * kNumCu);
             }
             break;
-        case 6: launch_block<8192, 1024, V>(A, B, rows, nrows, crpt, ccol, cval, s); break;
+        case 6: launch_block<8192, 1024, V>(A, B, rl, crpt, ccol, cval, s, kNumCu); break;
         default:
-            k_num_rank<V><<<nrows, kLongBS, RankWin<V>::lds, s>>>(
-                A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
-                B.N, crpt, ccol, cval, ls);
+            k_num_rank<V><<<row_grid(rl, kNumCu), kLongBS, RankWin<V>::lds, s>>>(
+                A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, B.N, crpt,
+                ccol, cval, ls);
             break;
     }
 }
 
-template void launch_numeric_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
+template void launch_numeric_bin<double>(int, const CsrIn &, const CsrIn &, const RowList &, const long long *, int *,
                                          double *, const LongScratch &, int *, int, hipStream_t);
-template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *, int *,
+template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const RowList &, const long long *, int *,
                                         float *, const LongScratch &, int *, int, hipStream_t);
 
 namespace {

@@ -36,6 +36,20 @@ __global__ void k_zero_stats(DevStats *st)
     for (unsigned i = threadIdx.x; i < sizeof(DevStats) / 4; i += blockDim.x) p[i] = 0;
 }
 
+// info[0] = nnz(C), info[1] = 1 when it exceeds `capacity` (0 otherwise);
+// on overflow every numeric bin is emptied, so device-sized numeric kernels
+// write nothing.
+__global__ void k_capacity(DevStats *st, long long capacity, long long *info)
+{
+    if (threadIdx.x != 0) return;
+    const long long nnz = static_cast<long long>(st->total_nnz);
+    const bool over = nnz > capacity;
+    info[0] = nnz;
+    info[1] = over ? 1 : 0;
+    if (over)
+        for (int b = 0; b < kNumBins; b++) st->num_size[b] = 0;
+}
+
 // Products per row, grid-stride over tiles of 256 rows.  The block walks
 // the tile's A-nonzeros flat (one contiguous range of A.col): in each round
 // thread t takes 4 consecutive entries, so every lane has four independent
@@ -399,6 +413,11 @@ void launch_fill_int(int *p, int v, long long n, hipStream_t s)
 }
 
 void launch_zero_stats(DevStats *st, hipStream_t s) { k_zero_stats<<<1, 256, 0, s>>>(st); }
+
+void launch_capacity_check(DevStats *st, long long capacity, long long *info, hipStream_t s)
+{
+    k_capacity<<<1, 64, 0, s>>>(st, capacity, info);
+}
 
 void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, const BinBounds &bb,
                   hipStream_t s)

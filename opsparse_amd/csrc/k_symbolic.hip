@@ -73,7 +73,7 @@ struct BitmapInsert {
 template <int G, int T, int BS, int H, bool BITMAP>
 __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                  const int *__restrict__ brpt, const int *__restrict__ bcol,
-                                                 const int *__restrict__ rows, int nrows, int *__restrict__ row_nnz)
+                                                 const RowList rl, int *__restrict__ row_nnz)
 {
     constexpr int U = G < kWave ? G : kWave;
     constexpr int RPB = BS / G;
@@ -85,62 +85,68 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     const int tid = threadIdx.x;
     const int slot = tid / G;
     const int unit = (tid % G) / U;
-    for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(tab)[i] = make_int4(-1, -1, -1, -1);
-    if (NU > 1 && tid < RPB) s_cnt[tid] = 0;
-    __syncthreads();
+    const int *rows;
+    const int nrows = row_list(rl, rows);
+    const int ntiles = (nrows + RPB - 1) / RPB;
+    for (int tile = xcd_tile(); tile < ntiles; tile += gridDim.x) {
+        for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(tab)[i] = make_int4(-1, -1, -1, -1);
+        if (NU > 1 && tid < RPB) s_cnt[tid] = 0;
+        __syncthreads();
 
-    const int ridx = xcd_tile() * RPB + slot;
-    int cnt = 0;
-    int r = 0;
-    if (ridx < nrows) {
-        r = rows ? rows[ridx] : ridx;
-        const int a_beg = arpt[r], a_end = arpt[r + 1];
-        const FullRow range{brpt};
-        int *stab = tab + slot * T;
-        bool narrow = false;
-        if constexpr (NU == 1 && BITMAP) {
-            // A row whose products span fewer than 32*T columns sets bits
-            // in the slot's T words (an OR per product, no CAS, no
-            // probing) and counts them by popcount.
-            const int gl = tid % G;
-            int mn = INT_MAX, mx = -1;
-            for (int e = a_beg + gl; e < a_end; e += U) {
-                const int k = acol[e];
-                const int bs = brpt[k], be = brpt[k + 1];
-                if (bs < be) {
-                    mn = min(mn, bcol[bs]);
-                    mx = max(mx, bcol[be - 1]);
+        const int ridx = tile * RPB + slot;
+        int cnt = 0;
+        int r = 0;
+        if (ridx < nrows) {
+            r = rows ? rows[ridx] : ridx;
+            const int a_beg = arpt[r], a_end = arpt[r + 1];
+            const FullRow range{brpt};
+            int *stab = tab + slot * T;
+            bool narrow = false;
+            if constexpr (NU == 1 && BITMAP) {
+                // A row whose products span fewer than 32*T columns sets bits
+                // in the slot's T words (an OR per product, no CAS, no
+                // probing) and counts them by popcount.
+                const int gl = tid % G;
+                int mn = INT_MAX, mx = -1;
+                for (int e = a_beg + gl; e < a_end; e += U) {
+                    const int k = acol[e];
+                    const int bs = brpt[k], be = brpt[k + 1];
+                    if (bs < be) {
+                        mn = min(mn, bcol[bs]);
+                        mx = max(mx, bcol[be - 1]);
+                    }
+                }
+                mn = unit_min<U>(mn);
+                mx = unit_max<U>(mx);
+                narrow = mx >= 0 && mx - mn < 32 * T;
+                if (narrow) {
+                    const int nw = ((mx - mn) >> 5) + 1;
+                    for (int i = gl; i < nw; i += U) stab[i] = 0;
+                    wave_lds_sync();
+                    SymBitmap op{bcol, reinterpret_cast<unsigned *>(stab), mn};
+                    for_each_product<U, false, double, true>(a_beg, a_end, unit, NU, acol,
+                                                             static_cast<const double *>(nullptr), range, op);
+                    wave_lds_sync();
+                    for (int i = gl; i < nw; i += U) cnt += __popc(static_cast<unsigned>(stab[i]));
                 }
             }
-            mn = unit_min<U>(mn);
-            mx = unit_max<U>(mx);
-            narrow = mx >= 0 && mx - mn < 32 * T;
-            if (narrow) {
-                const int nw = ((mx - mn) >> 5) + 1;
-                for (int i = gl; i < nw; i += U) stab[i] = 0;
-                wave_lds_sync();
-                SymBitmap op{bcol, reinterpret_cast<unsigned *>(stab), mn};
+            if (!narrow) {
+                SymInsert<T, H> op{bcol, stab};
                 for_each_product<U, false, double, true>(a_beg, a_end, unit, NU, acol,
                                                          static_cast<const double *>(nullptr), range, op);
-                wave_lds_sync();
-                for (int i = gl; i < nw; i += U) cnt += __popc(static_cast<unsigned>(stab[i]));
+                cnt = op.cnt;
             }
         }
-        if (!narrow) {
-            SymInsert<T, H> op{bcol, stab};
-            for_each_product<U, false, double, true>(a_beg, a_end, unit, NU, acol,
-                                                     static_cast<const double *>(nullptr), range, op);
-            cnt = op.cnt;
+        if constexpr (NU == 1) {
+            cnt = unit_sum<U>(cnt);
+            if ((tid % G) == 0 && ridx < nrows) row_nnz[r] = cnt;
+        } else {
+            cnt = unit_sum<kWave>(cnt);
+            if (lane_id() == 0) atomicAdd(&s_cnt[0], cnt);
+            __syncthreads();
+            if (tid == 0 && ridx < nrows) row_nnz[r] = s_cnt[0];
         }
-    }
-    if constexpr (NU == 1) {
-        cnt = unit_sum<U>(cnt);
-        if ((tid % G) == 0 && ridx < nrows) row_nnz[r] = cnt;
-    } else {
-        cnt = unit_sum<kWave>(cnt);
-        if (lane_id() == 0) atomicAdd(&s_cnt[0], cnt);
-        __syncthreads();
-        if (tid == 0 && ridx < nrows) row_nnz[r] = s_cnt[0];
+        __syncthreads();  // the next tile reinitialises the tables
     }
 }
 
@@ -190,8 +196,8 @@ struct WindowRange {
 
 __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const int *__restrict__ brpt, const int *__restrict__ bcol,
-                                                      const int *__restrict__ rows, int nrows, int ncols,
-                                                      int wcols, int *__restrict__ row_nnz, SymSave save)
+                                                      const RowList rl, int ncols, int wcols,
+                                                      int *__restrict__ row_nnz, SymSave save)
 {
     extern __shared__ unsigned bm[];
     __shared__ int s_red[kLongBS / kWave];
@@ -199,6 +205,8 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
     __shared__ ChunkStage<kLongBS, double> s_bst;
     static_assert(sizeof(unsigned) * kBitmapWords + sizeof(ChunkStage<kLongBS, double>) + 256 <= kMaxLds, "LDS");
     const int tid = threadIdx.x;
+    const int *rows;
+    const int nrows = row_list(rl, rows);
     // The bitmap is cleared once here and then by each window's count pass.
     for (int i = tid; i < (wcols >> 5); i += kLongBS) bm[i] = 0u;
     __syncthreads();
@@ -266,28 +274,28 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
 // Short-row bins (G <= 64) take the hash family chosen by the tuning; wide
 // rows (bins 5, 6) always hash with the Fibonacci family.
 template <int G, int T, int BS>
-void launch_hash(const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash, bool bitmap,
+void launch_hash(const CsrIn &A, const CsrIn &B, const RowList &rl, int *row_nnz, int hash, bool bitmap,
                  hipStream_t s)
 {
     constexpr int RPB = BS / G;
-    const int grid = xcd_grid((nrows + RPB - 1) / RPB);
+    const int grid = tile_grid(rl, RPB, kResidentThreads / BS);
     if constexpr (G <= kWave) {
         if (bitmap) {
             if (hash == kHashLin)
                 k_sym_hash<G, T, BS, kHashLin, true>
-                    <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+                    <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rl, row_nnz);
             else
                 k_sym_hash<G, T, BS, kHashFib, true>
-                    <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+                    <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rl, row_nnz);
             return;
         }
         if (hash == kHashLin) {
             k_sym_hash<G, T, BS, kHashLin, false>
-                <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+                <<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rl, row_nnz);
             return;
         }
     }
-    k_sym_hash<G, T, BS, kHashFib, false><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rows, nrows, row_nnz);
+    k_sym_hash<G, T, BS, kHashFib, false><<<grid, BS, 0, s>>>(A.rpt, A.col, B.rpt, B.col, rl, row_nnz);
 }
 
 }  // namespace
@@ -300,25 +308,25 @@ void symbolic_init_attributes()
                               static_cast<int>(symbolic_long_lds()));
 }
 
-void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash,
+void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, int *row_nnz, int hash,
                          bool bitmap, const SymSave &save, hipStream_t s)
 {
-    if (nrows <= 0) return;
+    if (rl.n <= 0) return;
     switch (bin) {
-        case 0: launch_hash<16, 32, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
-        case 1: launch_hash<32, 64, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
-        case 2: launch_hash<64, 128, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
-        case 3: launch_hash<64, 512, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
-        case 4: launch_hash<64, 1024, 256>(A, B, rows, nrows, row_nnz, hash, bitmap, s); break;
-        case 5: launch_hash<256, 4096, 256>(A, B, rows, nrows, row_nnz, hash, false, s); break;
-        case 6: launch_hash<1024, 16384, 1024>(A, B, rows, nrows, row_nnz, hash, false, s); break;
+        case 0: launch_hash<16, 32, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
+        case 1: launch_hash<32, 64, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
+        case 2: launch_hash<64, 128, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
+        case 3: launch_hash<64, 512, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
+        case 4: launch_hash<64, 1024, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
+        case 5: launch_hash<256, 4096, 256>(A, B, rl, row_nnz, hash, false, s); break;
+        case 6: launch_hash<1024, 16384, 1024>(A, B, rl, row_nnz, hash, false, s); break;
         default: {
             // Bitmap sized to the column count (up to 2^20 columns per window),
             // so narrow products keep more blocks resident per CU.
             const int words = static_cast<int>(std::min<long long>(kBitmapWords, (static_cast<long long>(B.N) + 31) / 32));
             const int wcols = words * 32;
-            k_sym_long<<<nrows, kLongBS, sizeof(unsigned) * static_cast<size_t>(words), s>>>(
-                A.rpt, A.col, B.rpt, B.col, rows, nrows, B.N, wcols, row_nnz,
+            k_sym_long<<<row_grid(rl, kNumCu), kLongBS, sizeof(unsigned) * static_cast<size_t>(words), s>>>(
+                A.rpt, A.col, B.rpt, B.col, rl, B.N, wcols, row_nnz,
                 B.N <= kSymLongWindow ? save : SymSave{});
             break;
         }

@@ -83,6 +83,37 @@ struct CsrIn {
     int M, N;
 };
 
+// Rows one per-bin kernel processes.  Host-sized (dsize == nullptr): the
+// list rows[0..n) (rows == nullptr: rows 0..n-1).  Device-sized: the list
+// starts at rows + dsize[0] + ... + dsize[bin - 1] and holds dsize[bin]
+// entries, read on the device (the bin counts a setup or scan kernel left in
+// DevStats, or a counter an earlier kernel appended rows under); n is then
+// only an upper bound that sizes the grid, and every kernel loops over its
+// tiles, so a launch recorded in a graph serves any bin sizes.
+struct RowList {
+    const int *rows = nullptr;
+    int n = 0;
+    const int *dsize = nullptr;
+    int bin = 0;
+};
+
+// Grid of a per-bin kernel over tiles of `per_tile` rows.  Host-sized: one
+// block per tile (a multiple of 8, for the XCD-aware tile order).
+// Device-sized: at most `max_blocks` blocks, each looping over tiles.
+constexpr int kNumCu = 256;
+constexpr int kResidentThreads = kNumCu * 2048;  // threads resident on the device at full occupancy
+inline int tile_grid(const RowList &rl, int per_tile, int max_blocks)
+{
+    long long tiles = (static_cast<long long>(rl.n) + per_tile - 1) / per_tile;
+    if (rl.dsize && tiles > max_blocks) tiles = max_blocks;
+    return static_cast<int>((tiles + 7) / 8 * 8);
+}
+// Grid of a kernel whose blocks take rows blockIdx.x, blockIdx.x + gridDim.x, ...
+inline int row_grid(const RowList &rl, int max_blocks)
+{
+    return rl.dsize ? (rl.n < max_blocks ? rl.n : max_blocks) : rl.n;
+}
+
 // --- setup / binning / scan (k_setup_scan.hip) ---
 void launch_zero_stats(DevStats *st, hipStream_t s);
 // row_flop[M] (int32, saturated), per-bin counts; rows with f == 0 get row_nnz = 0.
@@ -98,6 +129,10 @@ int scan_blocks(int M);
 void launch_scan(const int *row_nnz, const int *row_flop, const int *arpt, int M, long long *block_sums,
                  long long *crpt64, int *crpt32, DevStats *st, const BinBounds &bb, hipStream_t s);
 void launch_narrow_rpt(const long long *crpt64, int M, int *crpt32, hipStream_t s);
+// After the scan and the numeric binning of a device-sized call: info[0] =
+// nnz(C), info[1] = 1 if nnz(C) > capacity (then every numeric bin size in
+// st is set to 0, so no numeric kernel writes), else 0.
+void launch_capacity_check(DevStats *st, long long capacity, long long *info, hipStream_t s);
 // p[0..n) = v with a kernel (no memset node when captured in a graph).
 void launch_fill_int(int *p, int v, long long n, hipStream_t s);
 
@@ -117,11 +152,11 @@ struct SymSave {
     int min_nnz = 0;
     int cap = 0;
 };
-// Computes row_nnz for the rows of one bin.  rows == nullptr means rows 0..nrows-1.
+// Computes row_nnz for the rows of one bin.
 // hash: family for the short-row bins 0-4 (kHashLin or kHashFib in device.hpp).
 // bitmap: bins 0-4 count rows with a narrow column span in a bitmap.
 // save: bitmaps of bin 7 kept for the numeric phase (only when B.N <= kSymLongWindow).
-void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, int *row_nnz, int hash,
+void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, int *row_nnz, int hash,
                          bool bitmap, const SymSave &save, hipStream_t s);
 // Bytes of dynamic LDS the long-row symbolic kernel asks for.
 size_t symbolic_long_lds();
@@ -157,8 +192,8 @@ void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s);
 
 // spill: scratch of (rows of bin kSpillBin + 1) ints, or nullptr (block kernel only).
 template <typename V>
-void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
-                        int *ccol, V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s);
+void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, const long long *crpt, int *ccol,
+                        V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s);
 // One-time kernel attribute setup (dynamic LDS limits).
 void numeric_init_attributes();
 void symbolic_init_attributes();

@@ -20,9 +20,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
+#include "batch.hpp"
 #include "launch.hpp"
 #include "opsparse_amd.h"
 #include "outmem.hpp"
@@ -381,7 +384,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
 
     t0 = now_s();
     dispatch_bins(ws, s, st.sym_size, ev, [&](int b, hipStream_t ps) {
-        launch_symbolic_bin(b, A, B, sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b], ws->row_nnz.p,
+        launch_symbolic_bin(b, A, B, RowList{sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b]}, ws->row_nnz.p,
                             pc.hash, ws->sym_bitmap, save, ps);
     });
     if (sep) {
@@ -443,7 +446,7 @@ void numeric_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, const Plan
 {
     const bool ev = t && t->kernel_events;
     dispatch_bins(ws, s, pc.num_size, ev, [&](int b, hipStream_t ps) {
-        launch_numeric_bin<V>(b, A, B, pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b], pc.crpt64, ccol,
+        launch_numeric_bin<V>(b, A, B, RowList{pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b]}, pc.crpt64, ccol,
                               cval, pc.ls, pc.spill, pc.hash, ps);
     });
 }
@@ -767,6 +770,216 @@ int spgemm_noalloc_impl(const ops_csr *A, const ops_csr *B, ops_csr64 *C, int64_
     numeric_scratch(pc, A, B, ws.cursor, ws.spill, ws.wcut, s, false);
     numeric_phase<V>(&ws, a, b, pc, C->col, static_cast<V *>(C->val), s, nullptr);
     CK(hipStreamSynchronize(s));
+    return OPS_OK;
+}
+
+}  // namespace
+
+// ---- whole calls with device-side sizes (ops_exec_*) ----------------------
+//
+// Every step of a call is issued on one stream without a host
+// synchronisation: bin sizes, bin offsets and nnz(C) stay on the device.
+// Each per-bin kernel is launched for every bin at a width set by the row
+// count and reads its bin's size and offset from DevStats (RowList with
+// dsize), looping over its tiles; C goes to buffers of fixed capacity, and a
+// check between the scan and the numeric phase empties every numeric bin
+// when nnz(C) exceeds it (the launch then writes no entry and the overflow
+// is reported).  The sequence is captured once into a hipGraph and replayed:
+// any patterns of A and B with the same descriptors (pointers, sizes, entry
+// counts) give a correct product.  A batched executor adds the packing of
+// the A_i and B_i in front (pack kernels over fixed instance tables) and
+// the shift of each C_i back to its own columns behind.
+struct ops_exec {
+    ops_workspace *ws = nullptr;
+    ops_dtype dt = OPS_F64;
+    bool use_graph = true;
+    ops_csr A{}, B{};  // the product's operands (the packed arrays for a batch)
+    // batch: instance tables, packed operand arrays and the bases of C_i
+    bool batch = false;
+    int count = 0;
+    PackedSide pa, pb;
+    DevBuf<int> a_rpt, a_col, b_rpt, b_col, rbase, cbase;
+    DevBuf<double> a_val, b_val;  // FP32 batches use the first half of each
+    // scratch of the sequence (its own: a graph keeps these addresses)
+    DevBuf<int> row_flop, row_nnz, bins, spill, cursor, wcut;
+    DevBuf<long long> block_sums, info;
+    DevStats *stats = nullptr;
+    int span = 0, nwin = 0;
+    bool cut = false;
+    // C: row offsets (M + 1), columns and values with room for `capacity`
+    long long capacity = 0;
+    DevBuf<long long> crpt;
+    DevBuf<int> ccol;
+    DevBuf<double> cval;
+    hipGraph_t src = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// The whole call of `ex` on stream s (no host synchronisation).
+void exec_sequence(ops_exec *ex, hipStream_t s)
+{
+    ops_workspace *ws = ex->ws;
+    const bool f64 = ex->dt == OPS_F64;
+    if (ex->batch) {
+        pack_side_launch(ex->pa, f64, ex->a_rpt.p, ex->a_col.p, ex->a_val.p, s);
+        pack_side_launch(ex->pb, f64, ex->b_rpt.p, ex->b_col.p, ex->b_val.p, s);
+        CK(hipGetLastError());
+    }
+    const CsrIn A = as_in(&ex->A), B = as_in(&ex->B);
+    const int M = A.M;
+    BinBounds bb = ws->bounds;
+    if (B.N <= kSymLongWindow) bb.sym[kNumBins - 2] = bb.sym[kNumBins - 3];
+    DevStats *st = ex->stats;
+    const int all[kNumBins] = {1, 1, 1, 1, 1, 1, 1, 1};
+    launch_zero_stats(st, s);
+    if (M > 0) {
+        launch_setup(A, B, ex->row_flop.p, ex->row_nnz.p, st, bb, s);
+        launch_bin_scatter(ex->row_flop.p, M, 0, st, ex->bins.p, bb, s);
+        CK(hipGetLastError());
+        dispatch_bins(ws, s, all, false, [&](int b, hipStream_t ps) {
+            launch_symbolic_bin(b, A, B, RowList{ex->bins.p, M, st->sym_size, b}, ex->row_nnz.p, ws->short_hash,
+                                ws->sym_bitmap, SymSave{}, ps);
+        });
+    }
+    launch_scan(ex->row_nnz.p, ex->row_flop.p, A.rpt, M, ex->block_sums.p, ex->crpt.p, nullptr, st, bb, s);
+    if (M > 0) launch_bin_scatter(ex->row_nnz.p, M, 1, st, ex->bins.p, bb, s);
+    launch_capacity_check(st, ex->capacity, ex->info.p, s);
+    CK(hipGetLastError());
+    if (M > 0) {
+        LongScratch ls;
+        ls.cursor = ex->cursor.p;
+        ls.span = ex->span;
+        if (ex->cut) {
+            launch_window_cuts(B, ex->nwin, ex->wcut.p, s);
+            CK(hipGetLastError());
+            ls.wcut = ex->wcut.p;
+            ls.nwin = ex->nwin;
+        }
+        dispatch_bins(ws, s, all, false, [&](int b, hipStream_t ps) {
+            const RowList rl{ex->bins.p, M, st->num_size, b};
+            if (f64)
+                launch_numeric_bin<double>(b, A, B, rl, ex->crpt.p, ex->ccol.p, ex->cval.p, ls, ex->spill.p,
+                                           ws->short_hash, ps);
+            else
+                launch_numeric_bin<float>(b, A, B, rl, ex->crpt.p, ex->ccol.p, reinterpret_cast<float *>(ex->cval.p),
+                                          ls, ex->spill.p, ws->short_hash, ps);
+        });
+    }
+    if (ex->batch) {
+        launch_unshift(ex->crpt.p, ex->ccol.p, ex->rbase.p, ex->cbase.p, ex->count, s);
+        CK(hipGetLastError());
+    }
+}
+
+// Scratch of the sequence, sized from the operand descriptors (and, for the
+// long-row bin, from A's last entry index, read once here).
+void exec_scratch(ops_exec *ex, hipStream_t s)
+{
+    const int M = ex->A.M;
+    ex->row_flop.ensure(static_cast<size_t>(M) + 1);
+    ex->row_nnz.ensure(static_cast<size_t>(M) + 1);
+    ex->bins.ensure(static_cast<size_t>(M) + 1);
+    ex->spill.ensure(static_cast<size_t>(M) + 1);
+    ex->block_sums.ensure(static_cast<size_t>(scan_blocks(M)) + 2);
+    ex->info.ensure(2);
+    const long long zero[2] = {0, 0};  // nothing launched yet: nnz 0, no overflow
+    CK(hipMemcpyAsync(ex->info.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
+    CK(hipStreamSynchronize(s));
+    ex->crpt.ensure(static_cast<size_t>(M) + 1);
+    if (!ex->stats) CK(hipMalloc(&ex->stats, sizeof(DevStats)));
+    // For a batch the packed A's last entry index is its entry count; a
+    // descriptor is read on the device (its rpt[M]).
+    ex->span = ex->batch ? ex->A.nnz : static_cast<int>(a_index_span(&ex->A, s));
+    ex->cursor.ensure(4 * static_cast<size_t>(ex->span) + 4);
+    ex->nwin = rank_windows(ex->B.N);
+    ex->cut = cut_table_wanted(ex->B.M, ex->B.N);
+    if (ex->cut) ex->wcut.ensure(static_cast<size_t>(ex->B.M) * (ex->nwin + 1));
+}
+
+void exec_set_capacity(ops_exec *ex, long long cap)
+{
+    ex->capacity = cap;
+    const size_t n = static_cast<size_t>(std::max<long long>(cap, 1));
+    ex->ccol.ensure(n);
+    // FP32 values use the first half of the FP64-sized buffer's bytes
+    ex->cval.ensure(ex->dt == OPS_F64 ? n : (n + 1) / 2);
+}
+
+// Reads nnz(C) and the overflow flag of the last completed sequence.
+void exec_info(ops_exec *ex, long long &nnz, bool &over, hipStream_t s)
+{
+    long long h[2] = {0, 0};
+    CK(hipMemcpyAsync(h, ex->info.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    nnz = h[0];
+    over = h[1] != 0;
+}
+
+// Sizes C: capacity > 0 is taken as given; otherwise one eager sequence
+// with no room measures nnz(C) and C gets exactly that.  Then the graph is
+// captured (on the workspace's capture stream) when asked for.
+void exec_finish(ops_exec *ex, long long capacity, hipStream_t s)
+{
+    exec_scratch(ex, s);
+    if (capacity <= 0) {
+        exec_set_capacity(ex, 0);
+        exec_sequence(ex, s);
+        long long nnz = 0;
+        bool over = false;
+        exec_info(ex, nnz, over, s);
+        capacity = nnz;
+    }
+    exec_set_capacity(ex, capacity);
+    if (!ex->use_graph) return;
+    CK(hipStreamSynchronize(s));
+    hipStream_t cs = ex->ws->cap;
+    CK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+        exec_sequence(ex, cs);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(cs, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    CK(hipStreamEndCapture(cs, &ex->src));
+    CK(hipGraphInstantiate(&ex->exec, ex->src, nullptr, nullptr, 0));
+}
+
+void exec_release(ops_exec *ex)
+{
+    if (ex->exec) (void)hipGraphExecDestroy(ex->exec);
+    if (ex->src) (void)hipGraphDestroy(ex->src);
+    if (ex->stats) (void)hipFree(ex->stats);
+    ex->pa.release();
+    ex->pb.release();
+    delete ex;
+}
+
+template <typename F>
+int exec_create(ops_workspace *ws, ops_dtype dt, int use_graph, ops_exec **out, F &&fill)
+{
+    if (!ws || !out) return OPS_INVALID;
+    if (dt != OPS_F64 && dt != OPS_F32) return OPS_INVALID;
+    *out = nullptr;
+    ops_exec *ex = new (std::nothrow) ops_exec;
+    if (!ex) return OPS_OOM;
+    ex->ws = ws;
+    ex->dt = dt;
+    ex->use_graph = use_graph != 0;
+    const int st = guarded(ws, [&]() {
+        CK(hipSetDevice(ws->device));
+        init_attrs();
+        fill(ex);
+        return OPS_OK;
+    });
+    if (st != OPS_OK) {
+        exec_release(ex);
+        return st;
+    }
+    *out = ex;
     return OPS_OK;
 }
 
@@ -1178,6 +1391,106 @@ int ops_plan_numeric_graph(ops_plan *plan, ops_dtype dt, const ops_csr *A, const
         CK(hipGraphLaunch(plan->graph, s));
         return OPS_OK;
     });
+}
+
+int ops_exec_create(ops_workspace *ws, ops_dtype dt, const ops_csr *A, const ops_csr *B, int64_t capacity,
+                    int use_graph, ops_exec **out)
+{
+    if (!A || !B) return OPS_INVALID;
+    return exec_create(ws, dt, use_graph, out, [&](ops_exec *ex) {
+        validate(A, B);
+        ex->A = *A;
+        ex->B = *B;
+        exec_finish(ex, capacity, ws->cap);
+    });
+}
+
+int ops_exec_create_batch(ops_workspace *ws, ops_dtype dt, int32_t count, const ops_csr *A, const ops_csr *B,
+                          int64_t capacity, int use_graph, int64_t *row_base, int64_t *col_base, ops_exec **out)
+{
+    if (count < 0 || (count > 0 && (!A || !B))) return OPS_INVALID;
+    return exec_create(ws, dt, use_graph, out, [&](ops_exec *ex) {
+        const hipStream_t s = ws->cap;
+        for (int i = 0; i < count; i++)
+            if (A[i].N != B[i].M) throw HipFail(OPS_NONSQUARE, "instance with A_i.N != B_i.M");
+        std::vector<int64_t> ra(static_cast<size_t>(count) + 1), ca(static_cast<size_t>(count) + 1),
+            rb(static_cast<size_t>(count) + 1), cb(static_cast<size_t>(count) + 1);
+        int st = pack_side_prepare(count, A, ex->pa, ra.data(), ca.data(), s);
+        if (st == OPS_OK) st = pack_side_prepare(count, B, ex->pb, rb.data(), cb.data(), s);
+        if (st != OPS_OK) throw HipFail(st, "batch instances");
+        ex->batch = true;
+        ex->count = count;
+        const size_t wv = dt == OPS_F64 ? 1 : 2;  // values per double of storage
+        ex->a_rpt.ensure(static_cast<size_t>(ex->pa.rows) + 1);
+        ex->a_col.ensure(std::max<size_t>(ex->pa.nnz, 1));
+        ex->a_val.ensure(std::max<size_t>((ex->pa.nnz + wv - 1) / wv, 1));
+        ex->b_rpt.ensure(static_cast<size_t>(ex->pb.rows) + 1);
+        ex->b_col.ensure(std::max<size_t>(ex->pb.nnz, 1));
+        ex->b_val.ensure(std::max<size_t>((ex->pb.nnz + wv - 1) / wv, 1));
+        ex->A = ops_csr{ex->pa.rows, ex->pa.cols, ex->pa.nnz, ex->a_rpt.p, ex->a_col.p, ex->a_val.p};
+        ex->B = ops_csr{ex->pb.rows, ex->pb.cols, ex->pb.nnz, ex->b_rpt.p, ex->b_col.p, ex->b_val.p};
+        // C_i: rows of A_i, columns of B_i
+        std::vector<int> hr(static_cast<size_t>(count) + 1), hc(static_cast<size_t>(count) + 1);
+        for (int i = 0; i <= count; i++) {
+            hr[i] = static_cast<int>(ra[i]);
+            hc[i] = static_cast<int>(cb[i]);
+        }
+        ex->rbase.ensure(hr.size());
+        ex->cbase.ensure(hc.size());
+        CK(hipMemcpyAsync(ex->rbase.p, hr.data(), sizeof(int) * hr.size(), hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(ex->cbase.p, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice, s));
+        CK(hipStreamSynchronize(s));
+        if (row_base) std::memcpy(row_base, ra.data(), sizeof(int64_t) * ra.size());
+        if (col_base) std::memcpy(col_base, cb.data(), sizeof(int64_t) * cb.size());
+        // the operands must be packed before the sizing pass reads them
+        pack_side_launch(ex->pa, dt == OPS_F64, ex->a_rpt.p, ex->a_col.p, ex->a_val.p, s);
+        pack_side_launch(ex->pb, dt == OPS_F64, ex->b_rpt.p, ex->b_col.p, ex->b_val.p, s);
+        CK(hipGetLastError());
+        validate(&ex->A, &ex->B);
+        exec_finish(ex, capacity, s);
+    });
+}
+
+int ops_exec_launch(ops_exec *ex, void *stream)
+{
+    if (!ex) return OPS_INVALID;
+    return guarded(ex->ws, [&]() {
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (ex->exec)
+            CK(hipGraphLaunch(ex->exec, s));
+        else
+            exec_sequence(ex, s);
+        return OPS_OK;
+    });
+}
+
+int ops_exec_result(ops_exec *ex, ops_csr64 *C, int64_t *nnz_c, void *stream)
+{
+    if (!ex) return OPS_INVALID;
+    return guarded(ex->ws, [&]() {
+        long long nnz = 0;
+        bool over = false;
+        exec_info(ex, nnz, over, static_cast<hipStream_t>(stream));
+        if (nnz_c) *nnz_c = nnz;
+        if (C) {
+            *C = ops_csr64{};
+            C->M = ex->A.M;
+            C->N = ex->B.N;
+            C->nnz = nnz;
+            C->rpt = reinterpret_cast<int64_t *>(ex->crpt.p);
+            C->col = ex->ccol.p;
+            C->val = ex->cval.p;
+        }
+        return over ? OPS_INVALID : OPS_OK;
+    });
+}
+
+int ops_exec_destroy(ops_exec *ex)
+{
+    if (!ex) return OPS_INVALID;
+    (void)hipDeviceSynchronize();
+    exec_release(ex);
+    return OPS_OK;
 }
 
 }  // extern "C"

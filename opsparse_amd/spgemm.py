@@ -18,8 +18,8 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from ._lib import (OPS_F32, OPS_F64, OPS_OK, OpsCsr, OpsCsr64, OpsError, OpsMemStats, OpsTimings, OpsTuning, check,
-                   lib)
+from ._lib import (OPS_F32, OPS_F64, OPS_INVALID, OPS_OK, OpsCsr, OpsCsr64, OpsError, OpsMemStats, OpsTimings,
+                   OpsTuning, check, lib)
 from .gen import HostCSR
 
 _HIP = None
@@ -175,10 +175,11 @@ class LibCSR:
     """C allocated by the library (opsparse_spgemm semantics); freed with
     ops_csr_free / ops_csr64_free when this object dies."""
 
-    def __init__(self, desc, wide: bool, dtype: torch.dtype):
+    def __init__(self, desc, wide: bool, dtype: torch.dtype, owned: bool = True):
         self.d = desc
         self.wide = wide
         self.dtype = dtype
+        self.owned = owned  # False: a view of arrays another object owns (never freed here)
 
     @property
     def M(self):
@@ -253,7 +254,8 @@ class LibCSR:
 
     def free(self):
         if self.d is not None:
-            (lib().ops_csr64_free if self.wide else lib().ops_csr_free)(C.byref(self.d))
+            if self.owned:
+                (lib().ops_csr64_free if self.wide else lib().ops_csr_free)(C.byref(self.d))
             self.d = None
 
     def __del__(self):
@@ -325,6 +327,70 @@ def batch_pack(instances, dtype=torch.float64, stream=None):
     check(lib().ops_batch_pack(n, arr, dtype_code(dtype), C.byref(packed), rb.ctypes.data_as(C.POINTER(C.c_int64)),
                                cb.ctypes.data_as(C.POINTER(C.c_int64)), _stream_ptr(stream)))
     return LibCSR(packed, False, dtype), rb, cb
+
+
+class Executor:
+    """The whole call C = A*B replayed from a hipGraph (ops_exec_*): setup,
+    binning, symbolic, scan and numeric with bin sizes and nnz(C) on the
+    device, no host synchronisation per launch.  The operands' descriptors
+    are fixed; their patterns and values may change between launches (same
+    entry counts).  ``Executor.batch`` packs independent instances A_i, B_i
+    at every launch and hands each C_i back in its own columns (rows
+    ``row_base[i]:row_base[i+1]`` of the result)."""
+
+    def __init__(self, A: DeviceCSR | None = None, B: DeviceCSR | None = None, dtype=torch.float64,
+                 ws: Workspace | None = None, capacity: int = 0, graph: bool = True, _batch=None):
+        _require_gpu()
+        self.ws = ws or default_workspace()
+        self.dtype = dtype
+        self.ptr = C.c_void_p()
+        self.row_base = self.col_base = None
+        if _batch is None:
+            self._keep = (A, B)  # the graph holds their device addresses
+            a, b = A.desc(), B.desc()
+            check(lib().ops_exec_create(self.ws.ptr, dtype_code(dtype), C.byref(a), C.byref(b), int(capacity),
+                                        int(graph), C.byref(self.ptr)), self.ws.ptr)
+        else:
+            As, Bs = _batch
+            self._keep = (list(As), list(Bs))
+            n = len(As)
+            arr_a, arr_b = (OpsCsr * max(n, 1))(), (OpsCsr * max(n, 1))()
+            for i in range(n):
+                arr_a[i], arr_b[i] = As[i].desc(), Bs[i].desc()
+            rb = np.zeros(n + 1, dtype=np.int64)
+            cb = np.zeros(n + 1, dtype=np.int64)
+            check(lib().ops_exec_create_batch(self.ws.ptr, dtype_code(dtype), n, arr_a, arr_b, int(capacity),
+                                              int(graph), rb.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              cb.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(self.ptr)), self.ws.ptr)
+            self.row_base, self.col_base = rb, cb
+
+    @classmethod
+    def batch(cls, As, Bs, dtype=torch.float64, ws: Workspace | None = None, capacity: int = 0, graph: bool = True):
+        return cls(dtype=dtype, ws=ws, capacity=capacity, graph=graph, _batch=(As, Bs))
+
+    def launch(self, stream=None) -> None:
+        check(lib().ops_exec_launch(self.ptr, _stream_ptr(stream)), self.ws.ptr)
+
+    def result(self, stream=None) -> tuple[int, LibCSR]:
+        """(status, C view) of the last launch (waits for it).  status is
+        OPS_OK, or OPS_INVALID when nnz(C) exceeded the capacity (C.nnz is
+        then the size needed and no entry was written)."""
+        c, nnz = OpsCsr64(), C.c_int64()
+        st = lib().ops_exec_result(self.ptr, C.byref(c), C.byref(nnz), _stream_ptr(stream))
+        if st not in (OPS_OK, OPS_INVALID):
+            check(st, self.ws.ptr)
+        return st, LibCSR(c, True, self.dtype, owned=False)
+
+    def destroy(self) -> None:
+        if self.ptr:
+            lib().ops_exec_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class Plan:
@@ -434,7 +500,7 @@ def spgemm_autograd(a_val: torch.Tensor, b_val: torch.Tensor, A: DeviceCSR, B: D
     return plan, val, col
 
 
-__all__ = ["DeviceCSR", "Workspace", "LibCSR", "Plan", "spgemm", "spgemm_safe", "backward", "SpGEMMFunction",
+__all__ = ["DeviceCSR", "Workspace", "LibCSR", "Executor", "Plan", "spgemm", "spgemm_safe", "backward", "SpGEMMFunction",
            "spgemm_autograd", "default_workspace", "OpsTimings", "OpsError", "dtype_code", "batch_pack", "compute_flop_device",
            "default_tuning", "tuning_caps", "mem_cache_stats", "set_mem_cache_limit", "mem_cache_release",
            "set_kept_bitmap_limit"]

@@ -1,0 +1,152 @@
+"""Whole calls with device-side sizes, replayed from a hipGraph (ops_exec_*).
+
+Every launch is checked against the CPU oracle: structure exact, values
+within the FP64 / FP32 tolerance.  The graph is replayed several times
+with eager library work (plain calls) and host synchronisations between
+the replays, and the operands' values and patterns change between
+replays (same descriptors, same entry counts), so each replay must take
+its bin sizes from the device.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import opsparse_amd.gen as G
+from opsparse_amd._lib import OPS_INVALID, OPS_OK
+from opsparse_amd.spgemm import DeviceCSR, Executor, spgemm
+from helpers import TOL, csr_from_rows, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def check_result(ex, A, B, dtype=np.float64):
+    st, C = ex.result()
+    assert st == OPS_OK, st
+    rpt, col, val = C.to_host()
+    Ar = A.astype(dtype) if dtype == np.float32 else A
+    Br = B.astype(dtype) if dtype == np.float32 else B
+    return oracle.compare(rpt, col, val, oracle.spgemm(Ar, Br), TOL[dtype])
+
+
+def refill(dX, X):
+    """Writes host CSR X (same M, N and entry count) into dX's arrays."""
+    dX.rpt.copy_(torch.from_numpy(np.ascontiguousarray(X.rpt)))
+    dX.col.copy_(torch.from_numpy(np.ascontiguousarray(X.col)))
+    dX.val.copy_(torch.from_numpy(np.ascontiguousarray(X.val)).to(dX.val.dtype))
+
+
+def shuffled_rows(X, rng):
+    """Same M, N, nnz; rows' lengths permuted and new columns and values."""
+    lens = np.diff(X.rpt)
+    rng.shuffle(lens)
+    return random_csr(X.M, X.N, lens, rng)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_exec_every_bin_replays(gpu, graph):
+    rng = np.random.default_rng(41)
+    A = G.rmat(13, 16, 7)
+    dA = DeviceCSR.from_host(A, torch.float64)
+    nnz0 = len(oracle.spgemm(A, A)[1])
+    ex = Executor(dA, dA, torch.float64, capacity=2 * nnz0, graph=graph)
+    A1 = G.HostCSR(A.M, A.N, A.rpt, A.col, rng.uniform(-1, 1, A.col.size))
+    for step, X in enumerate((A, A, A1, shuffled_rows(A, rng))):
+        refill(dA, X)
+        ex.launch()
+        e = check_result(ex, X, X)
+        log(f"step {step}: nnz {e['nnz']}, max_norm {e['max_norm']:.2e}")
+        spgemm(dA, dA).free()  # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_exec_long_rows_spill_and_cut_table(gpu):
+    """Long rows over several column windows with B's window-cut table, the
+    wave kernel of bin 5 and its spill list, short bins; FP64 and FP32."""
+    rng = np.random.default_rng(16)
+    W = 1 << 18
+    N = 5 * W + 7
+    Kw, Kn = 1100, 100
+    wide = random_csr(Kw, 3 * W, rng.integers(250, 330, Kw), rng)
+    col = wide.col.astype(np.int64)
+    col = np.array([0, 2, 5], dtype=np.int64)[col // W] * W + col % W
+    rows = [np.minimum(col[wide.rpt[i]:wide.rpt[i + 1]], N - 1) for i in range(Kw)]
+    rows += [np.sort(rng.choice(np.arange(1000, 9000), size=int(rng.integers(60, 100)), replace=False))
+             for _ in range(Kn)]
+    B = csr_from_rows(Kw + Kn, N, rows, rng)
+    a_rows = ([np.sort(rng.choice(Kw, size=int(rng.integers(300, 420)), replace=False)) for _ in range(3)]
+              + [np.sort(rng.choice(np.arange(Kw, Kw + Kn), size=int(rng.integers(6, 10)), replace=False))
+                 for _ in range(3)]
+              + [np.sort(rng.choice(Kw, size=int(rng.integers(2, 4)), replace=False)) for _ in range(3)]
+              + [np.array([int(rng.integers(0, Kw))]) for _ in range(3)])
+    A = csr_from_rows(len(a_rows), Kw + Kn, a_rows, rng)
+    for dt, npdt in ((torch.float64, np.float64), (torch.float32, np.float32)):
+        dA, dB = DeviceCSR.from_host(A, dt), DeviceCSR.from_host(B, dt)
+        ex = Executor(dA, dB, dt)
+        for scale in (1.0, -0.5):
+            A2 = G.HostCSR(A.M, A.N, A.rpt, A.col, A.val * scale)
+            refill(dA, A2)
+            for _ in range(2):
+                ex.launch()
+                check_result(ex, A2, B, npdt)
+        ex.destroy()
+
+
+def test_exec_capacity_overflow_writes_nothing(gpu):
+    A = G.rmat(11, 8, 3)
+    dA = DeviceCSR.from_host(A, torch.float64)
+    ref = oracle.spgemm(A, A)
+    nnz = len(ref[1])
+    ex = Executor(dA, dA, torch.float64, capacity=nnz - 1)
+    ex.launch()
+    st, C = ex.result()
+    assert st == OPS_INVALID and C.nnz == nnz, (st, C.nnz)
+    assert np.array_equal(C.rowptr_to_host(), ref[0])
+    ex2 = Executor(dA, dA, torch.float64, capacity=nnz)
+    ex2.launch()
+    check_result(ex2, A, A)
+
+
+@pytest.mark.parametrize("dt,npdt", [(torch.float64, np.float64), (torch.float32, np.float32)])
+def test_exec_batch(gpu, dt, npdt):
+    rng = np.random.default_rng(5)
+    n = 96
+    As, Bs = [], []
+    for i in range(n):
+        m = int(np.exp(rng.uniform(np.log(16), np.log(3000))))
+        k = int(np.exp(rng.uniform(np.log(16), np.log(3000))))
+        nc = int(np.exp(rng.uniform(np.log(16), np.log(3000))))
+        if i % 17 == 5:
+            m = 0  # instances with no rows are skipped
+        As.append(random_csr(m, k, rng.integers(1, 33, m), rng))
+        Bs.append(random_csr(k, nc, rng.integers(1, 33, k), rng))
+    dAs = [DeviceCSR.from_host(X, dt) for X in As]
+    dBs = [DeviceCSR.from_host(X, dt) for X in Bs]
+    # room for any patterns with these entry counts: nnz(C_i) <= nnz(A_i) * longest B_i row
+    cap = sum(int(X.col.size) * int(np.diff(Y.rpt).max(initial=0)) for X, Y in zip(As, Bs))
+    ex = Executor.batch(dAs, dBs, dt, capacity=cap)
+    rb = ex.row_base
+    for rep in range(3):
+        if rep == 2:  # new values and patterns of every A_i (same entry counts)
+            for i in range(n):
+                if As[i].M:
+                    As[i] = shuffled_rows(As[i], rng)
+                    refill(dAs[i], As[i])
+        ex.launch()
+        st, C = ex.result()
+        assert st == OPS_OK
+        worst = 0.0
+        for i in range(n):
+            rpt, col, val = C.rows_to_host(int(rb[i]), int(rb[i + 1]))
+            Ar = As[i].astype(npdt) if npdt == np.float32 else As[i]
+            Br = Bs[i].astype(npdt) if npdt == np.float32 else Bs[i]
+            e = oracle.compare(rpt, col, val, oracle.spgemm(Ar, Br), TOL[npdt])
+            worst = max(worst, e["max_norm"])
+        log(f"batch rep {rep}: {n} instances, nnz(C) {C.nnz}, worst max_norm {worst:.2e}")
+        spgemm(dAs[0], dBs[0]).free()
+        torch.cuda.synchronize()
+    ex.destroy()
