@@ -13,9 +13,12 @@ as a nested record (config.stencil_100).
 Other workloads (--workload):
   stencil  : the stencil alone
   uniform  : 4096^2, 8 distinct uniform columns per row, FP64, seed 0
-  batch    : 4096 independent C_i = A_i*B_i, one fused autotuned call (and
-             a replayed whole-call hipGraph) against the plain call looped
-             over the instances; FP64 line, FP32 in the config
+  batch    : 4096 independent C_i = A_i*B_i, one fused autotuned call
+             (also timed with the packing of the instances inside), the
+             whole call (packing, product, shift of each C_i to its own
+             columns) replayed from one hipGraph with bin sizes and nnz(C)
+             on the device, against the plain call looped over the
+             instances; FP64 line, FP32 in the config
   backward : R-MAT 2^18 A*B in FP32, forward + masked backward, gradients
              checked against FP64 direct sums and FP64 finite differences
 
@@ -348,15 +351,38 @@ def nested_record(gflops, ms, cfg, roof, cpu):
 
 # ----------------------------------------------------------------------------- batched
 
+def batch_instances(dX, row_base, dtype):
+    """Descriptors of the instances of a packed square batch in their own
+    coordinates: instance i is rows [row_base[i], row_base[i+1]) of dX (a
+    row slice: its rpt starts at the instance's first entry) with columns
+    shifted back by row_base[i] (square instances: column bases = row bases)."""
+    import numpy as np
+    import torch
+    from opsparse_amd.spgemm import DeviceCSR
+    rb = torch.from_numpy(np.asarray(row_base, dtype=np.int64)).to(dX.rpt.device)
+    rows_inst = torch.repeat_interleave(torch.arange(len(row_base) - 1, device=rb.device), rb[1:] - rb[:-1])
+    ent_inst = torch.repeat_interleave(rows_inst, (dX.rpt[1:] - dX.rpt[:-1]).long())
+    local_col = (dX.col.long() - rb[ent_inst]).to(torch.int32)
+    ends = dX.rpt.cpu().numpy()
+    out = []
+    for i in range(len(row_base) - 1):
+        r0, r1 = int(row_base[i]), int(row_base[i + 1])
+        out.append(DeviceCSR(r1 - r0, r1 - r0, dX.rpt[r0:r1 + 1], local_col, dX.val, int(ends[r1] - ends[r0])))
+    return out, local_col
+
+
 def run_batch(args):
     """Independent C_i = A_i*B_i packed block-diagonally: one fused call
-    (and the replayed numeric hipGraph of its plan) against the plain call
-    looped over the instances (row-slice descriptors of the packed A)."""
+    (also with the packing of the instances inside the timed span), the
+    whole call replayed from a hipGraph with device-side bin sizes
+    (packing, product and the shift of each C_i to its own columns), and
+    the replayed numeric hipGraph of a plan, against the plain call looped
+    over the instances (row-slice descriptors of the packed A)."""
     import numpy as np
     import torch
     import opsparse_amd.gen as G
     from opsparse_amd import autotune
-    from opsparse_amd.spgemm import DeviceCSR, Plan, Workspace, spgemm
+    from opsparse_amd.spgemm import DeviceCSR, Executor, Plan, Workspace, batch_pack, spgemm
     t_gen = time.time()
     b = G.batch(args.batch_count, 3, 64, 16384, 4, 32)
     P = G.compute_flop_host(b.A, b.B)
@@ -389,6 +415,57 @@ def run_batch(args):
                     e = oracle.compare(rpt, col, val, oracle.spgemm_rows(Ah, Bh, s0, s1), 1e-12 if w == 8 else 1e-5)
                     err["max_norm"] = max(err["max_norm"], e["max_norm"])
         c_last.free()
+        # the instances as separate descriptors, packed inside the timed span
+        iA, colA = batch_instances(dA, b.row_base, dt)
+        iB, colB = batch_instances(dB, b.row_base, dt)
+        t_pack, t_packed_call = 0.0, 0.0
+        for rep in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pa, _, _ = batch_pack(iA, dt)
+            pb, _, _ = batch_pack(iB, dt)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            c = spgemm(pa, pb, ws=ws, wide=True)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if rep >= args.warmup:
+                t_pack += t1 - t0
+                t_packed_call += t2 - t0
+            c.free()
+            pa.free()
+            pb.free()
+        t_pack /= args.steps
+        t_packed_call /= args.steps
+        # the whole call (pack, product, column shift) replayed from one hipGraph
+        ex = Executor.batch(iA, iB, dt, ws=ws)
+        for _ in range(args.warmup):
+            ex.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ex.launch()
+        torch.cuda.synchronize()
+        t_exec = (time.perf_counter() - t0) / args.steps
+        st, cx = ex.result()
+        exec_err = None
+        if st != 0:
+            raise RuntimeError(f"batch executor status {st}")
+        if not args.no_check:
+            import oracle
+            exec_err = 0.0
+            for i in (0, args.batch_count // 2, args.batch_count - 1):
+                s0, s1 = int(b.row_base[i]), int(b.row_base[i + 1])
+                rpt, col, val = cx.rows_to_host(s0, s1)
+                ref = oracle.spgemm_rows(Ah, Bh, s0, s1)
+                # the oracle's columns are packed ones; the executor's are the instance's own
+                ref = (ref[0], ref[1] - s0, ref[2], ref[3])
+                e = oracle.compare(rpt, col, val, ref, 1e-12 if w == 8 else 1e-5)
+                exec_err = max(exec_err, e["max_norm"])
+        log(f"batch {tag}: pack {t_pack * 1e3:.3f} ms, pack + fused call {t_packed_call * 1e3:.3f} ms, "
+            f"whole-call graph {t_exec * 1e3:.3f} ms")
+        ex.destroy()
+        del iA, iB, colA, colB, cx
         # structure reuse: the numeric phase of the fused plan replayed as a hipGraph
         plan = Plan(dA, dB, ws=ws)
         col = torch.empty(max(plan.nnz, 1), dtype=torch.int32, device="cuda")
@@ -418,6 +495,7 @@ def run_batch(args):
             ci.free()
         log(f"batch {tag}: graph numeric {t_graph * 1e3:.3f} ms, looped plain call {t_loop * 1e3:.1f} ms")
         res[tag] = {"t_fused": t_fused, "t_graph": t_graph, "t_loop": t_loop, "tims": tims, "nnz_c": nnz_c,
+                    "t_pack": t_pack, "t_packed_call": t_packed_call, "t_exec": t_exec, "exec_err": exec_err,
                     "phases": phase_ms(tims),
                     "tuning": tune["tuning"], "cpu": cpu, "err": err}
         del dA, dB
@@ -429,11 +507,19 @@ def run_batch(args):
                        f"and run as one fused, autotuned call (FP64 line; FP32 under fp32_*)",
            "instances": args.batch_count, "rows": int(b.A.M), "nnz_a": int(b.A.nnz), "nnz_b": int(b.B.nnz),
            "products": int(P), "nnz_c": int(r["nnz_c"]), "gen_s": round(t_gen, 1),
-           "fused_ms": round(r["t_fused"] * 1e3, 3), "graph_numeric_replay_ms": round(r["t_graph"] * 1e3, 3),
+           "fused_ms": round(r["t_fused"] * 1e3, 3), "pack_ms": round(r["t_pack"] * 1e3, 3),
+           "pack_and_fused_ms": round(r["t_packed_call"] * 1e3, 3),
+           "whole_call_graph_ms": round(r["t_exec"] * 1e3, 3),
+           "speedup_whole_call_graph_vs_loop": round(r["t_loop"] / r["t_exec"], 2),
+           "speedup_pack_and_fused_vs_loop": round(r["t_loop"] / r["t_packed_call"], 2),
+           "graph_numeric_replay_ms": round(r["t_graph"] * 1e3, 3),
            "looped_plain_call_ms": round(r["t_loop"] * 1e3, 3),
            "speedup_fused_vs_loop": round(r["t_loop"] / r["t_fused"], 2),
            "speedup_graph_replay_vs_loop": round(r["t_loop"] / r["t_graph"], 2),
            "fp32_fused_ms": round(r32["t_fused"] * 1e3, 3),
+           "fp32_pack_and_fused_ms": round(r32["t_packed_call"] * 1e3, 3),
+           "fp32_whole_call_graph_ms": round(r32["t_exec"] * 1e3, 3),
+           "fp32_speedup_whole_call_graph_vs_loop": round(r32["t_loop"] / r32["t_exec"], 2),
            "fp32_gflops": round(2.0 * P / r32["t_fused"] / 1e9, 3),
            "fp32_graph_numeric_replay_ms": round(r32["t_graph"] * 1e3, 3),
            "fp32_looped_plain_call_ms": round(r32["t_loop"] * 1e3, 3),
@@ -442,7 +528,10 @@ def run_batch(args):
            "phases_f32": r32["phases"], "parallelism": "single GPU"}
     if r["err"] is not None:
         cfg["parity"] = {"structure": f"exact on instances 0..{n_cpu - 1}, middle and last",
-                         "max_norm_err_f64": r["err"]["max_norm"], "max_norm_err_f32": r32["err"]["max_norm"]}
+                         "max_norm_err_f64": r["err"]["max_norm"], "max_norm_err_f32": r32["err"]["max_norm"],
+                         "whole_call_graph": "structure exact (own columns) on instances 0, middle and last",
+                         "whole_call_graph_max_norm_err_f64": r["exec_err"],
+                         "whole_call_graph_max_norm_err_f32": r32["exec_err"]}
     return line(2.0 * P / r["t_fused"] / 1e9, args.steps, args.warmup, r["t_fused"] * 1e3, "f64", cfg, roof, r["cpu"])
 
 

@@ -260,14 +260,14 @@ typedef struct ops_exec ops_exec;
 int ops_exec_create(ops_workspace *ws, ops_dtype dt, const ops_csr *A, const ops_csr *B, int64_t capacity,
                     int use_graph, ops_exec **ex);
 /* Batched: C_i = A_i * B_i for `count` independent instances (host arrays
- * of device descriptors, as for ops_batch_pack).  Each launch packs the A_i
- * and the B_i block-diagonally (kernels over instance tables built here),
- * multiplies, and shifts every C_i back to its own columns: C_i is rows
- * [row_base[i], row_base[i+1]) of the executor's C, with B_i's column
- * indices.  row_base and col_base (count + 1 host entries, may be NULL)
- * receive the A_i row bases and the B_i column bases. */
+ * of device descriptors, as for ops_batch_pack; A_i.N == B_i.M).  Each
+ * launch packs the A_i block-diagonally and stacks the B_i (their columns
+ * unshifted) with kernels over instance tables built here, then multiplies:
+ * C_i is rows [row_base[i], row_base[i+1]) of the executor's C, in B_i's
+ * own column indices.  row_base (count + 1 host entries, may be NULL)
+ * receives the A_i row bases. */
 int ops_exec_create_batch(ops_workspace *ws, ops_dtype dt, int32_t count, const ops_csr *A, const ops_csr *B,
-                          int64_t capacity, int use_graph, int64_t *row_base, int64_t *col_base, ops_exec **ex);
+                          int64_t capacity, int use_graph, int64_t *row_base, ops_exec **ex);
 /* Enqueues one whole call on `stream` (returns at once). */
 int ops_exec_launch(ops_exec *ex, void *stream);
 /* Waits for `stream`, then describes the last launch's C (arrays owned by

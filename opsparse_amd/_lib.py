@@ -129,7 +129,7 @@ SIGNATURES = {
     "ops_batch_pack": (C.c_int, [C.c_int32, _P(OpsCsr), C.c_int, _P(OpsCsr), _P(C.c_int64), _P(C.c_int64), _vp]),
     "ops_exec_create": (C.c_int, [_vp, C.c_int, _P(OpsCsr), _P(OpsCsr), C.c_int64, C.c_int, _P(_vp)]),
     "ops_exec_create_batch": (C.c_int, [_vp, C.c_int, C.c_int32, _P(OpsCsr), _P(OpsCsr), C.c_int64, C.c_int,
-                                        _P(C.c_int64), _P(C.c_int64), _P(_vp)]),
+                                        _P(C.c_int64), _P(_vp)]),
     "ops_exec_launch": (C.c_int, [_vp, _vp]),
     "ops_exec_result": (C.c_int, [_vp, _P(OpsCsr64), _P(C.c_int64), _vp]),
     "ops_exec_destroy": (C.c_int, [_vp]),

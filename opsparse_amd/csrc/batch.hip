@@ -14,6 +14,7 @@
 // read and may be null).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -67,18 +68,6 @@ __global__ void k_set_last(int *prpt, int M, int nnz)
     if (threadIdx.x == 0) prpt[M] = nnz;
 }
 
-// Columns of C_i back to instance i's own coordinates: block i takes the
-// entries of rows [row_base[i], row_base[i+1]) and subtracts col_base[i].
-__global__ __launch_bounds__(256) void k_unshift(const long long *__restrict__ crpt, int *__restrict__ ccol,
-                                                 const int *__restrict__ row_base, const int *__restrict__ col_base)
-{
-    const int i = blockIdx.x;
-    const int cb = col_base[i];
-    if (cb == 0) return;
-    const long long e0 = crpt[row_base[i]], e1 = crpt[row_base[i + 1]];
-    for (long long e = e0 + threadIdx.x; e < e1; e += blockDim.x) ccol[e] -= cb;
-}
-
 int status_of(hipError_t e)
 {
     (void)hipGetLastError();
@@ -95,7 +84,7 @@ void PackedSide::release()
 }
 
 int pack_side_prepare(int count, const ops_csr *instances, PackedSide &side, int64_t *row_base, int64_t *col_base,
-                      hipStream_t s)
+                      hipStream_t s, bool stack)
 {
     side.release();
     side.rows = side.cols = side.nnz = 0;
@@ -149,12 +138,12 @@ int pack_side_prepare(int count, const ops_csr *instances, PackedSide &side, int
             if (n < 0 || n != c.nnz) return fail(OPS_INVALID);  // rpt[M] - rpt[0] must be the descriptor's nnz
             h[j].row_base = static_cast<int>(rows);
             h[j].nnz_base = nnz;
-            h[j].col_base = static_cast<int>(cols);
+            h[j].col_base = stack ? 0 : static_cast<int>(cols);
             j++;
             nnz += n;
         }
         rows += c.M;
-        cols += c.N;
+        cols = stack ? std::max<long long>(cols, c.N) : cols + c.N;
         if (rows > INT32_MAX || cols > INT32_MAX || nnz > INT32_MAX) return fail(OPS_INT32_OVERFLOW);
     }
     if (row_base) row_base[count] = rows;
@@ -183,12 +172,6 @@ void pack_side_launch(const PackedSide &side, bool f64, int *prpt, int *pcol, vo
             k_pack<float><<<side.nlive, 256, 0, s>>>(inst, prpt, pcol, static_cast<float *>(pval));
     }
     k_set_last<<<1, 64, 0, s>>>(prpt, side.rows, side.nnz);
-}
-
-void launch_unshift(const long long *crpt, int *ccol, const int *row_base, const int *col_base, int count,
-                    hipStream_t s)
-{
-    if (count > 0) k_unshift<<<count, 256, 0, s>>>(crpt, ccol, row_base, col_base);
 }
 
 }  // namespace opsamd

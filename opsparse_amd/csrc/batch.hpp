@@ -21,20 +21,15 @@ struct PackedSide {
 
 // Checks the host descriptors, reads each instance's rpt[0] and rpt[M] from
 // the device (synchronises `s`) and builds the table.  row_base and col_base
-// (count + 1 host entries, may be null) receive the packed bases.  Returns an
-// ops status.
+// (count + 1 host entries, may be null) receive the packed bases.  Packed
+// block-diagonally by default; `stack` keeps every instance's columns (all
+// column bases 0, as many columns as the widest instance).  Returns an ops
+// status.
 int pack_side_prepare(int count, const ops_csr *instances, PackedSide &side, int64_t *row_base, int64_t *col_base,
-                      hipStream_t s);
+                      hipStream_t s, bool stack = false);
 
 // Packs the side's instances into prpt (rows + 1), pcol and pval (nnz each)
 // with kernels only (no host synchronisation; may be captured in a graph).
 void pack_side_launch(const PackedSide &side, bool f64, int *prpt, int *pcol, void *pval, hipStream_t s);
-
-// Columns of a packed product back to each instance's own coordinates: for
-// instance i, the entries of rows [row_base[i], row_base[i+1]) of C (row
-// offsets crpt) lose col_base[i].  row_base and col_base are device arrays
-// of count + 1 entries.
-void launch_unshift(const long long *crpt, int *ccol, const int *row_base, const int *col_base, int count,
-                    hipStream_t s);
 
 }  // namespace opsamd

@@ -1050,6 +1050,49 @@ __device__ __forceinline__ int row_list(const RowList &rl, const int *&rows)
     return rl.dsize[rl.bin];
 }
 
+// Tile loops of per-bin kernels over a RowList: the first tile of this block
+// and the one after `cur`.  Host-sized lists (no counter) take tiles by a
+// fixed stride (XCD-aware for tiles, by block index for rows).  With a
+// counter, thread 0 takes `chunk` consecutive tiles at a time from it (about
+// eight takes per block: one atomic per tile would serialise the grid on a
+// single address) and `slot` (a __shared__ int) hands the first to the
+// block.  Every thread of the block calls these, with the block's LDS use
+// of the previous tile complete.
+struct TileSched {
+    int end = 0;    // end of the chunk being worked
+    int chunk = 1;  // tiles per take
+};
+__device__ __forceinline__ int xcd_tile();
+__device__ __forceinline__ int block_take(const RowList &rl, TileSched &ts, int *slot)
+{
+    __syncthreads();  // every thread has read the previous take from slot
+    if (threadIdx.x == 0) *slot = atomicAdd(rl.next, ts.chunk);
+    __syncthreads();
+    const int t = *slot;
+    ts.end = t + ts.chunk;
+    return t;
+}
+__device__ __forceinline__ int block_first(const RowList &rl, int ntiles, TileSched &ts, int *slot, int fixed)
+{
+    if (!rl.next) return fixed;
+    ts.chunk = max(1, ntiles / (static_cast<int>(gridDim.x) * 8));
+    return block_take(rl, ts, slot);
+}
+__device__ __forceinline__ int block_next(const RowList &rl, int cur, TileSched &ts, int *slot)
+{
+    if (!rl.next) return cur + static_cast<int>(gridDim.x);
+    if (cur + 1 < ts.end) return cur + 1;
+    return block_take(rl, ts, slot);
+}
+__device__ __forceinline__ int block_first_tile(const RowList &rl, int ntiles, TileSched &ts, int *slot)
+{
+    return block_first(rl, ntiles, ts, slot, rl.next ? 0 : xcd_tile());
+}
+__device__ __forceinline__ int block_first_row(const RowList &rl, int nrows, TileSched &ts, int *slot)
+{
+    return block_first(rl, nrows, ts, slot, static_cast<int>(blockIdx.x));
+}
+
 // Blocks are dispatched to the 8 XCDs round-robin (block b runs on XCD
 // b % 8).  With gridDim.x a multiple of 8, this returns a logical tile index
 // such that each XCD walks one contiguous range of tiles, so rows that share

@@ -65,6 +65,7 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     __shared__ __align__(16) int keys[RPB * T];
     __shared__ __align__(16) Acc vals[RPB * T];
     __shared__ int ckeys[RPB * NMAX];
+    __shared__ int s_tile;
 
     const int tid = threadIdx.x;
     const int slot = tid / G;
@@ -72,7 +73,9 @@ __global__ __launch_bounds__(BS) void k_num_small(const int *__restrict__ arpt, 
     const int *rows;
     const int nrows = row_list(rl, rows);
     const int ntiles = (nrows + RPB - 1) / RPB;
-    for (int tile = xcd_tile(); tile < ntiles; tile += gridDim.x) {
+    TileSched ts;
+    for (int tile = block_first_tile(rl, ntiles, ts, &s_tile); tile < ntiles;
+         tile = block_next(rl, tile, ts, &s_tile)) {
         for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(keys)[i] = make_int4(-1, -1, -1, -1);
         for (int i = tid; i < RPB * T * static_cast<int>(sizeof(Acc)) / 16; i += BS)
             reinterpret_cast<int4 *>(vals)[i] = make_int4(0, 0, 0, 0);
@@ -213,6 +216,7 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
     static_assert(sizeof(uint2) * T + sizeof(Acc) * (T / 2) <= sizeof(Acc) * T + sizeof(int) * (T + T / 2),
                   "rank path must fit in the hash path's LDS");
     __shared__ int s_pos;
+    __shared__ int s_tile;
     __shared__ int s_scan[BS / kWave + 1];
     __shared__ int s_mm[2 * (BS / kWave)];
     __shared__ BlockStage<BS, V> s_bst;
@@ -227,7 +231,9 @@ __global__ __launch_bounds__(BS) void k_num_block(const int *__restrict__ arpt, 
     const int lane = lane_id();
     const FullRow range{brpt};
 
-    for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
+    TileSched ts;
+    for (int ridx = block_first_row(rl, nrows, ts, &s_tile); ridx < nrows;
+         ridx = block_next(rl, ridx, ts, &s_tile)) {
         // The previous row's output stores complete here, so the walks'
         // loads are waited for one buffer at a time.
         drain_vmem();
@@ -388,6 +394,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
     constexpr int WPL = W / kWave;  // bitmap words per lane in the rank scan
     static_assert(W % kWave == 0 && NMAX * sizeof(Acc) >= NMAX * sizeof(int), "layout");
     __shared__ WaveRankLds<NMAX, W, V> s_lds[WPB];
+    __shared__ int s_tile;
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
     WaveRankLds<NMAX, W, V> &L = s_lds[wave];
@@ -447,7 +454,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
         const V cav = st.cav;
         if (mx < 0) return;  // no products: the row is empty
         if (mx - mn >= 32 * W) {
-            if (lane == 0) spill[1 + atomicAdd(spill, 1)] = r;
+            if (lane == 0) spill[2 + atomicAdd(spill, 1)] = r;
             return;
         }
         const int nw = ((mx - mn) >> 5) + 1;
@@ -502,10 +509,7 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
         for (int i = lane; i < n; i += kWave) ccol[off + i] = scol[i];
         wave_lds_sync();  // the next row rewrites this wave's LDS
     };
-    // Nothing below synchronises the block: each wave loops on its own.
-    for (int tile = xcd_tile(); tile < ntiles; tile += gridDim.x) {
-        const int ridx0 = (tile * WPB + wave) * 2;
-        if (ridx0 >= nrows) break;  // later tiles start further on
+    auto pair = [&](int ridx0) {
         const bool two = ridx0 + 1 < nrows;
         Start s0, s1;
         row_start(ridx0, s0);
@@ -514,12 +518,22 @@ __global__ __launch_bounds__(WPB * kWave) void k_num_wrank(const int *__restrict
         row_span_rest(s1);
         process(s0);
         if (two) process(s1);
+    };
+    // A tile is WPB pairs of rows, one per wave; the block takes tiles from
+    // the counter (device-sized lists, a barrier per tile) or by the
+    // XCD-aware stride.
+    TileSched ts;
+    for (int tile = block_first_tile(rl, ntiles, ts, &s_tile); tile < ntiles;
+         tile = block_next(rl, tile, ts, &s_tile)) {
+        const int ridx0 = (tile * WPB + wave) * 2;
+        if (ridx0 < nrows) pair(ridx0);
     }
 }
 
+// Clears a spill list's count and work counter.
 __global__ void k_zero_count(int *p)
 {
-    if (threadIdx.x == 0) p[0] = 0;
+    if (threadIdx.x < 2) p[threadIdx.x] = 0;
 }
 
 constexpr int kLongBS = 1024;
@@ -610,6 +624,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     __shared__ int s_scan[kLongBS / kWave + 1];
     __shared__ int s_min[kLongBS / kWave];
     __shared__ int s_cut;
+    __shared__ int s_tile;
     __shared__ BlockStage<kLongBS, V> s_bst;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
@@ -627,7 +642,9 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
     for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
     __syncthreads();
 
-    for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
+    TileSched ts;
+    for (int ridx = block_first_row(rl, nrows, ts, &s_tile); ridx < nrows;
+         ridx = block_next(rl, ridx, ts, &s_tile)) {
         const int r = rows ? rows[ridx] : ridx;
         const int a_beg = arpt[r], a_end = arpt[r + 1];
         // A row whose column bitmap the symbolic phase kept skips step 1:
@@ -908,8 +925,9 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
                                                  WPB * kWave, 0, s>>>(A.rpt, A.col, static_cast<const V *>(A.val),
                                                                       B.rpt, B.col, static_cast<const V *>(B.val), rl,
                                                                       crpt, ccol, cval, spill);
-                // the spilled rows: spill[1 .. spill[0]], counted on the device
-                const RowList sl{spill + 1, rl.n, spill, 0};
+                // the spilled rows: spill[2 .. 2 + spill[0]), counted on the
+                // device; the block kernel takes them from counter spill[1]
+                const RowList sl{spill + 2, rl.n, spill, 0, spill + 1};
                 launch_block<2048, 256, V>(A, B, sl, crpt, ccol, cval, s, 4096);
             } else {
                 launch_block<2048, 256, V>(A, B, rl, crpt, ccol, cval, s, 5 * kNumCu);

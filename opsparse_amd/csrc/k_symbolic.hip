@@ -81,6 +81,7 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     static_assert(BS % G == 0 && (T & (T - 1)) == 0, "bad config");
     __shared__ __align__(16) int tab[RPB * T];
     __shared__ int s_cnt[RPB];
+    __shared__ int s_tile;
 
     const int tid = threadIdx.x;
     const int slot = tid / G;
@@ -88,7 +89,9 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
     const int *rows;
     const int nrows = row_list(rl, rows);
     const int ntiles = (nrows + RPB - 1) / RPB;
-    for (int tile = xcd_tile(); tile < ntiles; tile += gridDim.x) {
+    TileSched ts;
+    for (int tile = block_first_tile(rl, ntiles, ts, &s_tile); tile < ntiles;
+         tile = block_next(rl, tile, ts, &s_tile)) {
         for (int i = tid; i < RPB * T / 4; i += BS) reinterpret_cast<int4 *>(tab)[i] = make_int4(-1, -1, -1, -1);
         if (NU > 1 && tid < RPB) s_cnt[tid] = 0;
         __syncthreads();
@@ -202,6 +205,7 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
     extern __shared__ unsigned bm[];
     __shared__ int s_red[kLongBS / kWave];
     __shared__ int s_slot;
+    __shared__ int s_tile;
     __shared__ ChunkStage<kLongBS, double> s_bst;
     static_assert(sizeof(unsigned) * kBitmapWords + sizeof(ChunkStage<kLongBS, double>) + 256 <= kMaxLds, "LDS");
     const int tid = threadIdx.x;
@@ -210,7 +214,9 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
     // The bitmap is cleared once here and then by each window's count pass.
     for (int i = tid; i < (wcols >> 5); i += kLongBS) bm[i] = 0u;
     __syncthreads();
-    for (int ridx = blockIdx.x; ridx < nrows; ridx += gridDim.x) {
+    TileSched ts;
+    for (int ridx = block_first_row(rl, nrows, ts, &s_tile); ridx < nrows;
+         ridx = block_next(rl, ridx, ts, &s_tile)) {
         const int r = rows ? rows[ridx] : ridx;
         const int a_beg = arpt[r], a_end = arpt[r + 1];
         int cnt = 0;

@@ -74,6 +74,9 @@ struct DevStats {
     unsigned long long num_annz[kNumBins];
     int kept_rows;  // long rows that asked for a kept column bitmap (SymSave::count)
     int pad_;
+    // work counters of device-sized launches (RowList::next), per bin
+    int next_sym[kNumBins];
+    int next_num[kNumBins];
 };
 
 struct CsrIn {
@@ -89,12 +92,16 @@ struct CsrIn {
 // entries, read on the device (the bin counts a setup or scan kernel left in
 // DevStats, or a counter an earlier kernel appended rows under); n is then
 // only an upper bound that sizes the grid, and every kernel loops over its
-// tiles, so a launch recorded in a graph serves any bin sizes.
+// tiles, so a launch recorded in a graph serves any bin sizes.  With
+// `next` (a counter zeroed before the launch), blocks take their tiles from
+// it one at a time instead of by a fixed stride, so a grid narrower than
+// the tile count stays balanced whatever the rows cost.
 struct RowList {
     const int *rows = nullptr;
     int n = 0;
     const int *dsize = nullptr;
     int bin = 0;
+    int *next = nullptr;
 };
 
 // Grid of a per-bin kernel over tiles of `per_tile` rows.  Host-sized: one
@@ -190,7 +197,8 @@ struct LongScratch {
 // kRankWindow (w = 0: the row start, w = nwin: the row end).
 void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s);
 
-// spill: scratch of (rows of bin kSpillBin + 1) ints, or nullptr (block kernel only).
+// spill: scratch of (rows of bin kSpillBin + 2) ints (a count, a work
+// counter, the rows), or nullptr (block kernel only).
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, const long long *crpt, int *ccol,
                         V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s);

@@ -384,8 +384,8 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
 
     t0 = now_s();
     dispatch_bins(ws, s, st.sym_size, ev, [&](int b, hipStream_t ps) {
-        launch_symbolic_bin(b, A, B, RowList{sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b]}, ws->row_nnz.p,
-                            pc.hash, ws->sym_bitmap, save, ps);
+        const RowList rl{sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b]};
+        launch_symbolic_bin(b, A, B, rl, ws->row_nnz.p, pc.hash, ws->sym_bitmap, save, ps);
     });
     if (sep) {
         CK(hipStreamSynchronize(s));
@@ -446,8 +446,8 @@ void numeric_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, const Plan
 {
     const bool ev = t && t->kernel_events;
     dispatch_bins(ws, s, pc.num_size, ev, [&](int b, hipStream_t ps) {
-        launch_numeric_bin<V>(b, A, B, RowList{pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b]}, pc.crpt64, ccol,
-                              cval, pc.ls, pc.spill, pc.hash, ps);
+        const RowList rl{pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b]};
+        launch_numeric_bin<V>(b, A, B, rl, pc.crpt64, ccol, cval, pc.ls, pc.spill, pc.hash, ps);
     });
 }
 
@@ -525,7 +525,7 @@ void numeric_scratch(PlanCore &pc, const ops_csr *A, const ops_csr *B, DevBuf<in
         }
     }
     if (pc.num_size[kSpillBin] > 0) {
-        spill.ensure(static_cast<size_t>(pc.num_size[kSpillBin]) + 1);
+        spill.ensure(static_cast<size_t>(pc.num_size[kSpillBin]) + 2);
         pc.spill = spill.p;
     }
 }
@@ -786,19 +786,29 @@ int spgemm_noalloc_impl(const ops_csr *A, const ops_csr *B, ops_csr64 *C, int64_
 // when nnz(C) exceeds it (the launch then writes no entry and the overflow
 // is reported).  The sequence is captured once into a hipGraph and replayed:
 // any patterns of A and B with the same descriptors (pointers, sizes, entry
-// counts) give a correct product.  A batched executor adds the packing of
-// the A_i and B_i in front (pack kernels over fixed instance tables) and
-// the shift of each C_i back to its own columns behind.
+// counts) give a correct product.
+//
+// A measuring pass at creation records which bins hold rows.  Those run
+// concurrently on the pool streams; the others are launched after the join,
+// on the main stream, where they find their bin empty and exit at once (or
+// do the work, if a later pattern fills them).  Launched concurrently, an
+// empty kernel whose blocks need much LDS would wait for a whole CU's LDS
+// and hold back the kernels queued behind it.
+//
+// A batched executor packs the A_i and B_i in front of the sequence (pack
+// kernels over fixed instance tables).  The A_i are packed block-diagonally
+// and the B_i stacked (rows one after the other, columns not shifted), so
+// the product's rows of instance i are A_i * B_i in B_i's own columns.
 struct ops_exec {
     ops_workspace *ws = nullptr;
     ops_dtype dt = OPS_F64;
     bool use_graph = true;
     ops_csr A{}, B{};  // the product's operands (the packed arrays for a batch)
-    // batch: instance tables, packed operand arrays and the bases of C_i
+    // batch: instance tables and packed operand arrays
     bool batch = false;
     int count = 0;
     PackedSide pa, pb;
-    DevBuf<int> a_rpt, a_col, b_rpt, b_col, rbase, cbase;
+    DevBuf<int> a_rpt, a_col, b_rpt, b_col;
     DevBuf<double> a_val, b_val;  // FP32 batches use the first half of each
     // scratch of the sequence (its own: a graph keeps these addresses)
     DevBuf<int> row_flop, row_nnz, bins, spill, cursor, wcut;
@@ -806,6 +816,8 @@ struct ops_exec {
     DevStats *stats = nullptr;
     int span = 0, nwin = 0;
     bool cut = false;
+    // bins holding rows in the measuring pass (symbolic, numeric)
+    int busy_sym[kNumBins] = {}, busy_num[kNumBins] = {};
     // C: row offsets (M + 1), columns and values with room for `capacity`
     long long capacity = 0;
     DevBuf<long long> crpt;
@@ -817,8 +829,23 @@ struct ops_exec {
 
 namespace {
 
-// The whole call of `ex` on stream s (no host synchronisation).
-void exec_sequence(ops_exec *ex, hipStream_t s)
+// Bins that held rows at creation run concurrently on the pool streams; the
+// rest follow on s after the join.
+template <typename F>
+void dispatch_device_sized(ops_workspace *ws, hipStream_t s, const int *busy, F &&launch)
+{
+    dispatch_bins(ws, s, busy, false, launch);
+    for (int b = kNumBins - 1; b >= 0; b--) {
+        if (busy[b] > 0) continue;
+        launch(b, s);
+        CK(hipGetLastError());
+    }
+}
+
+// The whole call of `ex` on stream s (no host synchronisation).  measure:
+// stop after the capacity check (no numeric phase), with every numeric bin
+// left as binned.
+void exec_sequence(ops_exec *ex, hipStream_t s, bool measure = false)
 {
     ops_workspace *ws = ex->ws;
     const bool f64 = ex->dt == OPS_F64;
@@ -832,45 +859,39 @@ void exec_sequence(ops_exec *ex, hipStream_t s)
     BinBounds bb = ws->bounds;
     if (B.N <= kSymLongWindow) bb.sym[kNumBins - 2] = bb.sym[kNumBins - 3];
     DevStats *st = ex->stats;
-    const int all[kNumBins] = {1, 1, 1, 1, 1, 1, 1, 1};
     launch_zero_stats(st, s);
     if (M > 0) {
         launch_setup(A, B, ex->row_flop.p, ex->row_nnz.p, st, bb, s);
         launch_bin_scatter(ex->row_flop.p, M, 0, st, ex->bins.p, bb, s);
         CK(hipGetLastError());
-        dispatch_bins(ws, s, all, false, [&](int b, hipStream_t ps) {
-            launch_symbolic_bin(b, A, B, RowList{ex->bins.p, M, st->sym_size, b}, ex->row_nnz.p, ws->short_hash,
-                                ws->sym_bitmap, SymSave{}, ps);
+        dispatch_device_sized(ws, s, ex->busy_sym, [&](int b, hipStream_t ps) {
+            launch_symbolic_bin(b, A, B, RowList{ex->bins.p, M, st->sym_size, b, st->next_sym + b}, ex->row_nnz.p,
+                                ws->short_hash, ws->sym_bitmap, SymSave{}, ps);
         });
     }
     launch_scan(ex->row_nnz.p, ex->row_flop.p, A.rpt, M, ex->block_sums.p, ex->crpt.p, nullptr, st, bb, s);
     if (M > 0) launch_bin_scatter(ex->row_nnz.p, M, 1, st, ex->bins.p, bb, s);
-    launch_capacity_check(st, ex->capacity, ex->info.p, s);
+    launch_capacity_check(st, measure ? LLONG_MAX : ex->capacity, ex->info.p, s);
     CK(hipGetLastError());
-    if (M > 0) {
-        LongScratch ls;
-        ls.cursor = ex->cursor.p;
-        ls.span = ex->span;
-        if (ex->cut) {
-            launch_window_cuts(B, ex->nwin, ex->wcut.p, s);
-            CK(hipGetLastError());
-            ls.wcut = ex->wcut.p;
-            ls.nwin = ex->nwin;
-        }
-        dispatch_bins(ws, s, all, false, [&](int b, hipStream_t ps) {
-            const RowList rl{ex->bins.p, M, st->num_size, b};
-            if (f64)
-                launch_numeric_bin<double>(b, A, B, rl, ex->crpt.p, ex->ccol.p, ex->cval.p, ls, ex->spill.p,
-                                           ws->short_hash, ps);
-            else
-                launch_numeric_bin<float>(b, A, B, rl, ex->crpt.p, ex->ccol.p, reinterpret_cast<float *>(ex->cval.p),
-                                          ls, ex->spill.p, ws->short_hash, ps);
-        });
-    }
-    if (ex->batch) {
-        launch_unshift(ex->crpt.p, ex->ccol.p, ex->rbase.p, ex->cbase.p, ex->count, s);
+    if (M == 0 || measure) return;
+    LongScratch ls;
+    ls.cursor = ex->cursor.p;
+    ls.span = ex->span;
+    if (ex->cut) {
+        launch_window_cuts(B, ex->nwin, ex->wcut.p, s);
         CK(hipGetLastError());
+        ls.wcut = ex->wcut.p;
+        ls.nwin = ex->nwin;
     }
+    dispatch_device_sized(ws, s, ex->busy_num, [&](int b, hipStream_t ps) {
+        const RowList rl{ex->bins.p, M, st->num_size, b, st->next_num + b};
+        if (f64)
+            launch_numeric_bin<double>(b, A, B, rl, ex->crpt.p, ex->ccol.p, ex->cval.p, ls, ex->spill.p,
+                                       ws->short_hash, ps);
+        else
+            launch_numeric_bin<float>(b, A, B, rl, ex->crpt.p, ex->ccol.p, reinterpret_cast<float *>(ex->cval.p), ls,
+                                      ex->spill.p, ws->short_hash, ps);
+    });
 }
 
 // Scratch of the sequence, sized from the operand descriptors (and, for the
@@ -881,12 +902,9 @@ void exec_scratch(ops_exec *ex, hipStream_t s)
     ex->row_flop.ensure(static_cast<size_t>(M) + 1);
     ex->row_nnz.ensure(static_cast<size_t>(M) + 1);
     ex->bins.ensure(static_cast<size_t>(M) + 1);
-    ex->spill.ensure(static_cast<size_t>(M) + 1);
+    ex->spill.ensure(static_cast<size_t>(M) + 2);
     ex->block_sums.ensure(static_cast<size_t>(scan_blocks(M)) + 2);
     ex->info.ensure(2);
-    const long long zero[2] = {0, 0};  // nothing launched yet: nnz 0, no overflow
-    CK(hipMemcpyAsync(ex->info.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
-    CK(hipStreamSynchronize(s));
     ex->crpt.ensure(static_cast<size_t>(M) + 1);
     if (!ex->stats) CK(hipMalloc(&ex->stats, sizeof(DevStats)));
     // For a batch the packed A's last entry index is its entry count; a
@@ -917,23 +935,26 @@ void exec_info(ops_exec *ex, long long &nnz, bool &over, hipStream_t s)
     over = h[1] != 0;
 }
 
-// Sizes C: capacity > 0 is taken as given; otherwise one eager sequence
-// with no room measures nnz(C) and C gets exactly that.  Then the graph is
-// captured (on the workspace's capture stream) when asked for.
+// The measuring pass (which bins hold rows, nnz(C)), then C's arrays:
+// `capacity` entries, or exactly nnz(C) when capacity <= 0.  Then the graph
+// is captured (on the workspace's capture stream) when asked for.
 void exec_finish(ops_exec *ex, long long capacity, hipStream_t s)
 {
     exec_scratch(ex, s);
-    if (capacity <= 0) {
-        exec_set_capacity(ex, 0);
-        exec_sequence(ex, s);
-        long long nnz = 0;
-        bool over = false;
-        exec_info(ex, nnz, over, s);
-        capacity = nnz;
-    }
-    exec_set_capacity(ex, capacity);
-    if (!ex->use_graph) return;
+    exec_sequence(ex, s, true);
+    DevStats h{};
+    CK(hipMemcpyAsync(&h, ex->stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
+    for (int b = 0; b < kNumBins; b++) {
+        ex->busy_sym[b] = h.sym_size[b];
+        ex->busy_num[b] = h.num_size[b];
+    }
+    if (capacity <= 0) capacity = static_cast<long long>(h.total_nnz);
+    exec_set_capacity(ex, capacity);
+    const long long zero[2] = {0, 0};  // nothing launched yet: nnz 0, no overflow
+    CK(hipMemcpyAsync(ex->info.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
+    CK(hipStreamSynchronize(s));
+    if (!ex->use_graph) return;
     hipStream_t cs = ex->ws->cap;
     CK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     try {
@@ -1406,17 +1427,16 @@ int ops_exec_create(ops_workspace *ws, ops_dtype dt, const ops_csr *A, const ops
 }
 
 int ops_exec_create_batch(ops_workspace *ws, ops_dtype dt, int32_t count, const ops_csr *A, const ops_csr *B,
-                          int64_t capacity, int use_graph, int64_t *row_base, int64_t *col_base, ops_exec **out)
+                          int64_t capacity, int use_graph, int64_t *row_base, ops_exec **out)
 {
     if (count < 0 || (count > 0 && (!A || !B))) return OPS_INVALID;
     return exec_create(ws, dt, use_graph, out, [&](ops_exec *ex) {
         const hipStream_t s = ws->cap;
         for (int i = 0; i < count; i++)
             if (A[i].N != B[i].M) throw HipFail(OPS_NONSQUARE, "instance with A_i.N != B_i.M");
-        std::vector<int64_t> ra(static_cast<size_t>(count) + 1), ca(static_cast<size_t>(count) + 1),
-            rb(static_cast<size_t>(count) + 1), cb(static_cast<size_t>(count) + 1);
-        int st = pack_side_prepare(count, A, ex->pa, ra.data(), ca.data(), s);
-        if (st == OPS_OK) st = pack_side_prepare(count, B, ex->pb, rb.data(), cb.data(), s);
+        // A_i block-diagonal (A_i's columns shifted to B_i's rows), B_i stacked
+        int st = pack_side_prepare(count, A, ex->pa, row_base, nullptr, s);
+        if (st == OPS_OK) st = pack_side_prepare(count, B, ex->pb, nullptr, nullptr, s, true);
         if (st != OPS_OK) throw HipFail(st, "batch instances");
         ex->batch = true;
         ex->count = count;
@@ -1429,23 +1449,6 @@ int ops_exec_create_batch(ops_workspace *ws, ops_dtype dt, int32_t count, const 
         ex->b_val.ensure(std::max<size_t>((ex->pb.nnz + wv - 1) / wv, 1));
         ex->A = ops_csr{ex->pa.rows, ex->pa.cols, ex->pa.nnz, ex->a_rpt.p, ex->a_col.p, ex->a_val.p};
         ex->B = ops_csr{ex->pb.rows, ex->pb.cols, ex->pb.nnz, ex->b_rpt.p, ex->b_col.p, ex->b_val.p};
-        // C_i: rows of A_i, columns of B_i
-        std::vector<int> hr(static_cast<size_t>(count) + 1), hc(static_cast<size_t>(count) + 1);
-        for (int i = 0; i <= count; i++) {
-            hr[i] = static_cast<int>(ra[i]);
-            hc[i] = static_cast<int>(cb[i]);
-        }
-        ex->rbase.ensure(hr.size());
-        ex->cbase.ensure(hc.size());
-        CK(hipMemcpyAsync(ex->rbase.p, hr.data(), sizeof(int) * hr.size(), hipMemcpyHostToDevice, s));
-        CK(hipMemcpyAsync(ex->cbase.p, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice, s));
-        CK(hipStreamSynchronize(s));
-        if (row_base) std::memcpy(row_base, ra.data(), sizeof(int64_t) * ra.size());
-        if (col_base) std::memcpy(col_base, cb.data(), sizeof(int64_t) * cb.size());
-        // the operands must be packed before the sizing pass reads them
-        pack_side_launch(ex->pa, dt == OPS_F64, ex->a_rpt.p, ex->a_col.p, ex->a_val.p, s);
-        pack_side_launch(ex->pb, dt == OPS_F64, ex->b_rpt.p, ex->b_col.p, ex->b_val.p, s);
-        CK(hipGetLastError());
         validate(&ex->A, &ex->B);
         exec_finish(ex, capacity, s);
     });

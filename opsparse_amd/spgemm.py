@@ -335,8 +335,8 @@ class Executor:
     device, no host synchronisation per launch.  The operands' descriptors
     are fixed; their patterns and values may change between launches (same
     entry counts).  ``Executor.batch`` packs independent instances A_i, B_i
-    at every launch and hands each C_i back in its own columns (rows
-    ``row_base[i]:row_base[i+1]`` of the result)."""
+    at every launch (A_i block-diagonally, B_i stacked) so that C_i, rows
+    ``row_base[i]:row_base[i+1]`` of the result, is in B_i's own columns."""
 
     def __init__(self, A: DeviceCSR | None = None, B: DeviceCSR | None = None, dtype=torch.float64,
                  ws: Workspace | None = None, capacity: int = 0, graph: bool = True, _batch=None):
@@ -344,7 +344,7 @@ class Executor:
         self.ws = ws or default_workspace()
         self.dtype = dtype
         self.ptr = C.c_void_p()
-        self.row_base = self.col_base = None
+        self.row_base = None
         if _batch is None:
             self._keep = (A, B)  # the graph holds their device addresses
             a, b = A.desc(), B.desc()
@@ -358,11 +358,10 @@ class Executor:
             for i in range(n):
                 arr_a[i], arr_b[i] = As[i].desc(), Bs[i].desc()
             rb = np.zeros(n + 1, dtype=np.int64)
-            cb = np.zeros(n + 1, dtype=np.int64)
             check(lib().ops_exec_create_batch(self.ws.ptr, dtype_code(dtype), n, arr_a, arr_b, int(capacity),
                                               int(graph), rb.ctypes.data_as(C.POINTER(C.c_int64)),
-                                              cb.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(self.ptr)), self.ws.ptr)
-            self.row_base, self.col_base = rb, cb
+                                              C.byref(self.ptr)), self.ws.ptr)
+            self.row_base = rb
 
     @classmethod
     def batch(cls, As, Bs, dtype=torch.float64, ws: Workspace | None = None, capacity: int = 0, graph: bool = True):

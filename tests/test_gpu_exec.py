@@ -150,3 +150,29 @@ def test_exec_batch(gpu, dt, npdt):
         spgemm(dAs[0], dBs[0]).free()
         torch.cuda.synchronize()
     ex.destroy()
+
+
+def test_exec_batch_sized_at_creation(gpu):
+    """capacity 0: creation runs one sequence with no room for C (the
+    capacity check empties the numeric bins and the column shift is skipped)
+    to measure nnz(C), then C gets exactly that."""
+    rng = np.random.default_rng(9)
+    n = 40
+    As = [random_csr(int(m), int(m), rng.integers(1, 20, int(m)), rng) for m in rng.integers(8, 600, n)]
+    Bs = [random_csr(X.N, int(rng.integers(8, 600)), rng.integers(1, 20, X.N), rng) for X in As]
+    dAs = [DeviceCSR.from_host(X, torch.float64) for X in As]
+    dBs = [DeviceCSR.from_host(X, torch.float64) for X in Bs]
+    ex = Executor.batch(dAs, dBs, torch.float64)
+    rb = ex.row_base
+    for rep in range(2):
+        if rep == 1:  # new values, same patterns
+            for i in range(n):
+                As[i] = G.HostCSR(As[i].M, As[i].N, As[i].rpt, As[i].col, rng.uniform(-1, 1, As[i].col.size))
+                refill(dAs[i], As[i])
+        ex.launch()
+        st, C = ex.result()
+        assert st == OPS_OK
+        for i in range(n):
+            rpt, col, val = C.rows_to_host(int(rb[i]), int(rb[i + 1]))
+            oracle.compare(rpt, col, val, oracle.spgemm(As[i], Bs[i]), TOL[np.float64])
+    ex.destroy()
