@@ -291,6 +291,23 @@ typedef struct ops_tuning {
     int32_t sym_bitmap;                  /* symbolic bins 0-4: 1 = a row whose products span fewer than
                                             32*T columns sets bits in its T-word table and counts them
                                             instead of hashing; 0 = always hash */
+    /* Kernel shape of bins 0-4 (0 = default; bins 5 and 6 take 0 only):
+     * lanes per row G, table slots T and block size BS.  No shape lowers a
+     * bin's capacity, so any shape goes with any valid bounds.
+     *   symbolic  bin: shape 0        shape 1        shape 2
+     *             0:   16/32/256      16/64/256      32/64/256
+     *             1:   32/64/256      32/128/256     64/128/256
+     *             2:   64/128/256     64/256/256     32/128/256
+     *             3:   64/512/256     64/1024/256    64/512/64
+     *             4:   64/1024/256    64/2048/256    64/1024/64
+     *   numeric   bin: shape 0        shape 1        shape 2       (G/T/BS)
+     *             0:   16/32/256      16/64/256      16/32/64
+     *             1:   32/64/256      32/128/256     16/64/256
+     *             2:   64/128/256     64/256/256     32/128/256
+     *             3:   32/256/64      64/256/64      32/512/64
+     *             4:   64/512/64      64/1024/64     64/512/256 */
+    int32_t sym_shape[OPS_NUM_BINS - 1];
+    int32_t num_shape[OPS_NUM_BINS - 1];
 } ops_tuning;
 
 int ops_tuning_default(ops_tuning *t);

@@ -58,11 +58,12 @@ class OpsTimings(C.Structure):
 
 class OpsTuning(C.Structure):
     _fields_ = [("sym_bound", C.c_int32 * (NUM_BINS - 1)), ("num_bound", C.c_int32 * (NUM_BINS - 1)),
-                ("short_hash", C.c_int32), ("sym_bitmap", C.c_int32)]
+                ("short_hash", C.c_int32), ("sym_bitmap", C.c_int32),
+                ("sym_shape", C.c_int32 * (NUM_BINS - 1)), ("num_shape", C.c_int32 * (NUM_BINS - 1))]
 
     def as_dict(self):
         return {"sym_bound": list(self.sym_bound), "num_bound": list(self.num_bound), "short_hash": self.short_hash,
-                "sym_bitmap": self.sym_bitmap}
+                "sym_bitmap": self.sym_bitmap, "sym_shape": list(self.sym_shape), "num_shape": list(self.num_shape)}
 
     @staticmethod
     def from_dict(d):
@@ -73,6 +74,10 @@ class OpsTuning(C.Structure):
             t.num_bound[i] = v
         t.short_hash = d["short_hash"]
         t.sym_bitmap = d.get("sym_bitmap", 0)
+        for i, v in enumerate(d.get("sym_shape", [0] * (NUM_BINS - 1))):
+            t.sym_shape[i] = v
+        for i, v in enumerate(d.get("num_shape", [0] * (NUM_BINS - 1))):
+            t.num_shape[i] = v
         return t
 
 

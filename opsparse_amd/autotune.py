@@ -1,4 +1,4 @@
-"""Autotuner for the SpGEMM bin boundaries and short-row hash family.
+"""Autotuner for the SpGEMM bin boundaries, hash family and kernel shapes.
 
 The reference fixes its bin bounds and hash at compile time
 (kernels/binning.cuh:21, 51, 105, 143; utils/config.hpp:32).  Here they are
@@ -13,10 +13,15 @@ Candidate space (every candidate respects the kernels' capacities):
   * symbolic bounds: default, or each bound halved (rows move to wider
     kernels with larger tables earlier);
   * numeric bounds: default, or each bound halved;
-  * symbolic bitmap for rows with a narrow column span: off or on.
+  * symbolic bitmap for rows with a narrow column span: off or on;
+then, from the best of those, one bin at a time (numeric bins 0-4, then
+symbolic bins 0-4, only bins that hold rows): the kernel shape, i.e. hash
+table size, lanes per row and block size (ops_tuning's shape table), kept
+when it is more than 2% faster.
 """
 from __future__ import annotations
 
+import copy
 import json
 import os
 import time
@@ -45,7 +50,8 @@ def candidates() -> list[dict]:
             for num_half in (False, True):
                 for sym_bitmap in (0, 1):
                     t = {"sym_bound": list(base["sym_bound"]), "num_bound": list(base["num_bound"]),
-                         "short_hash": short_hash, "sym_bitmap": sym_bitmap}
+                         "short_hash": short_hash, "sym_bitmap": sym_bitmap,
+                         "sym_shape": list(base["sym_shape"]), "num_shape": list(base["num_shape"])}
                     if sym_half:
                         t["sym_bound"] = [max(1, b // 2) for b in t["sym_bound"]]
                     if num_half:
@@ -54,6 +60,13 @@ def candidates() -> list[dict]:
                     assert all(v <= c for v, c in zip(t["num_bound"], caps["num_bound"]))
                     out.append(t)
     return out
+
+
+def _bins_in_use(A, B, ws: Workspace, wide: bool):
+    from ._lib import OpsTimings
+    t = OpsTimings()
+    spgemm(A, B, ws=ws, wide=wide, timings=t).free()
+    return [t.sym_rows[b] > 0 for b in range(5)], [t.num_rows[b] > 0 for b in range(5)]
 
 
 def _time_call(A, B, ws: Workspace, wide: bool, reps: int) -> float:
@@ -70,7 +83,7 @@ def _time_call(A, B, ws: Workspace, wide: bool, reps: int) -> float:
 
 
 def autotune(A, B, ws: Workspace | None = None, wide: bool = True, reps: int = 2, use_cache: bool = True,
-             verbose: bool = False) -> dict:
+             verbose: bool = False, shapes: bool = True) -> dict:
     """Picks and installs the fastest tuning for A*B on `ws`; returns
     {"tuning": ..., "seconds": ..., "from_cache": bool, "trials": [...]}."""
     from .spgemm import default_workspace
@@ -100,6 +113,23 @@ def autotune(A, B, ws: Workspace | None = None, wide: bool = True, reps: int = 2
     default = next(r for r in trials if r["tuning"] == default_tuning())
     if default["seconds"] <= best["seconds"] * 1.02:
         best = default
+    if shapes:
+        ws.set_tuning(best["tuning"])
+        sym_used, num_used = _bins_in_use(A, B, ws, wide)
+        for kind, used in (("num_shape", num_used), ("sym_shape", sym_used)):
+            for b in range(5):
+                if not used[b]:
+                    continue
+                for sh in range(1, 3):
+                    t = copy.deepcopy(best["tuning"])
+                    t[kind][b] = sh
+                    ws.set_tuning(t)
+                    sec = _time_call(A, B, ws, wide, reps)
+                    trials.append({"tuning": t, "seconds": sec})
+                    if verbose:
+                        print(f"  autotune {kind}[{b}]={sh}: {sec * 1e3:.3f} ms", flush=True)
+                    if sec < best["seconds"] * 0.98:
+                        best = {"tuning": t, "seconds": sec}
     ws.set_tuning(best["tuning"])
     if use_cache:
         cache[key] = {"tuning": best["tuning"], "seconds": best["seconds"]}

@@ -903,19 +903,35 @@ void numeric_init_attributes()
 
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, const long long *crpt, int *ccol,
-                        V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s)
+                        V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s, int shape)
 {
     if (rl.n <= 0) return;
+    // Bins 0-4 (G lanes per row, T slots, NMAX entries, BS threads), shape
+    // 0 the default, 1 twice the table, 2 other lanes per row or block size.
+    // Bin 3's default takes two rows per wave (32 lanes each), 4% faster
+    // than one row per wave on the 27-point stencil (the kernel is
+    // VALU-bound there, and the two rows share its per-step instructions);
+    // bin 4's, one row (one wave) per block, 2-3% faster than four rows per
+    // block.
+    switch (bin < 5 ? bin * kShapes + shape : -1) {
+        case 0: launch_small<16, 32, 16, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 1: launch_small<16, 64, 16, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 2: launch_small<16, 32, 16, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 3: launch_small<32, 64, 32, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 4: launch_small<32, 128, 32, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 5: launch_small<16, 64, 32, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 6: launch_small<64, 128, 64, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 7: launch_small<64, 256, 64, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 8: launch_small<32, 128, 64, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 9: launch_small<32, 256, 128, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 10: launch_small<64, 256, 128, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 11: launch_small<32, 512, 128, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 12: launch_small<64, 512, 256, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 13: launch_small<64, 1024, 256, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        case 14: launch_small<64, 512, 256, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); return;
+        default: break;
+    }
     switch (bin) {
-        case 0: launch_small<16, 32, 16, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
-        case 1: launch_small<32, 64, 32, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
-        case 2: launch_small<64, 128, 64, 256, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
-        // Bin 3: two rows per wave (32 lanes each), 4% faster than one row
-        // per wave on the 27-point stencil (the kernel is VALU-bound there,
-        // and the two rows share its per-step instructions).  Bin 4: one row
-        // (one wave) per block, 2-3% faster than four rows per block.
-        case 3: launch_small<32, 256, 128, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
-        case 4: launch_small<64, 512, 256, 64, V>(A, B, rl, crpt, ccol, cval, hash, s); break;
         case kSpillBin:
             if (spill) {
                 // Wave per row; rows with a wide column span go to the block kernel.
@@ -943,9 +959,9 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
 }
 
 template void launch_numeric_bin<double>(int, const CsrIn &, const CsrIn &, const RowList &, const long long *, int *,
-                                         double *, const LongScratch &, int *, int, hipStream_t);
+                                         double *, const LongScratch &, int *, int, hipStream_t, int);
 template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const RowList &, const long long *, int *,
-                                        float *, const LongScratch &, int *, int, hipStream_t);
+                                        float *, const LongScratch &, int *, int, hipStream_t, int);
 
 namespace {
 

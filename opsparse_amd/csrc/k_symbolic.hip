@@ -315,15 +315,30 @@ void symbolic_init_attributes()
 }
 
 void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, int *row_nnz, int hash,
-                         bool bitmap, const SymSave &save, hipStream_t s)
+                         bool bitmap, const SymSave &save, hipStream_t s, int shape)
 {
     if (rl.n <= 0) return;
+    // bins 0-4: shape 0 default, 1 twice the table, 2 other lanes per row or block size
+    const int k = bin < 5 ? bin * kShapes + shape : -1;
+    switch (k) {
+        case 0: launch_hash<16, 32, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 1: launch_hash<16, 64, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 2: launch_hash<32, 64, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 3: launch_hash<32, 64, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 4: launch_hash<32, 128, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 5: launch_hash<64, 128, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 6: launch_hash<64, 128, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 7: launch_hash<64, 256, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 8: launch_hash<32, 128, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 9: launch_hash<64, 512, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 10: launch_hash<64, 1024, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 11: launch_hash<64, 512, 64>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 12: launch_hash<64, 1024, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 13: launch_hash<64, 2048, 256>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        case 14: launch_hash<64, 1024, 64>(A, B, rl, row_nnz, hash, bitmap, s); return;
+        default: break;
+    }
     switch (bin) {
-        case 0: launch_hash<16, 32, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
-        case 1: launch_hash<32, 64, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
-        case 2: launch_hash<64, 128, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
-        case 3: launch_hash<64, 512, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
-        case 4: launch_hash<64, 1024, 256>(A, B, rl, row_nnz, hash, bitmap, s); break;
         case 5: launch_hash<256, 4096, 256>(A, B, rl, row_nnz, hash, false, s); break;
         case 6: launch_hash<1024, 16384, 1024>(A, B, rl, row_nnz, hash, false, s); break;
         default: {

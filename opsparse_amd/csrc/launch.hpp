@@ -163,8 +163,11 @@ struct SymSave {
 // hash: family for the short-row bins 0-4 (kHashLin or kHashFib in device.hpp).
 // bitmap: bins 0-4 count rows with a narrow column span in a bitmap.
 // save: bitmaps of bin 7 kept for the numeric phase (only when B.N <= kSymLongWindow).
+// shape: kernel shape of bins 0-4 (kShapes alternatives, 0 = default; the
+// table is in opsparse_amd.h, ops_tuning).
+constexpr int kShapes = 3;
 void launch_symbolic_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, int *row_nnz, int hash,
-                         bool bitmap, const SymSave &save, hipStream_t s);
+                         bool bitmap, const SymSave &save, hipStream_t s, int shape = 0);
 // Bytes of dynamic LDS the long-row symbolic kernel asks for.
 size_t symbolic_long_lds();
 
@@ -201,7 +204,7 @@ void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s);
 // counter, the rows), or nullptr (block kernel only).
 template <typename V>
 void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &rl, const long long *crpt, int *ccol,
-                        V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s);
+                        V *cval, const LongScratch &ls, int *spill, int hash, hipStream_t s, int shape = 0);
 // One-time kernel attribute setup (dynamic LDS limits).
 void numeric_init_attributes();
 void symbolic_init_attributes();

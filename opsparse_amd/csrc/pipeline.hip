@@ -140,6 +140,7 @@ struct ops_workspace {
     BinBounds bounds = default_bounds();
     int short_hash = kHashLin;
     bool sym_bitmap = false;
+    int sym_shape[kNumBins] = {}, num_shape[kNumBins] = {};  // kernel shapes of bins 0-4
     // Set only on the stack workspace of the allocation-free entry point:
     // every bin runs on the caller's stream, and neither kept column
     // bitmaps nor a window-cut table are made (both would allocate).
@@ -158,6 +159,7 @@ struct PlanCore {
     int num_size[kNumBins] = {};
     int num_off[kNumBins] = {};
     int hash = kHashLin;  // hash family of the short-row bins
+    int num_shape[kNumBins] = {};
     int kept_rows = 0;    // long rows whose symbolic column bitmap is kept (ls.bm)
     DevStats stats{};
 };
@@ -289,6 +291,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     pc.N = B.N;
     pc.crpt64 = crpt64_dst;
     pc.hash = ws->short_hash;
+    for (int b = 0; b < kNumBins; b++) pc.num_shape[b] = ws->num_shape[b];
     // When B's columns fit one bitmap window of the long-row symbolic
     // kernel, rows above the bin-5 bound go to it (bin 7) instead of the
     // 16384-slot hash of bin 6: setting bits costs no probing or CAS, and
@@ -385,7 +388,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
     t0 = now_s();
     dispatch_bins(ws, s, st.sym_size, ev, [&](int b, hipStream_t ps) {
         const RowList rl{sym_rows ? sym_rows + sym_off[b] : nullptr, st.sym_size[b]};
-        launch_symbolic_bin(b, A, B, rl, ws->row_nnz.p, pc.hash, ws->sym_bitmap, save, ps);
+        launch_symbolic_bin(b, A, B, rl, ws->row_nnz.p, pc.hash, ws->sym_bitmap, save, ps, ws->sym_shape[b]);
     });
     if (sep) {
         CK(hipStreamSynchronize(s));
@@ -447,7 +450,7 @@ void numeric_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, const Plan
     const bool ev = t && t->kernel_events;
     dispatch_bins(ws, s, pc.num_size, ev, [&](int b, hipStream_t ps) {
         const RowList rl{pc.bins ? pc.bins + pc.num_off[b] : nullptr, pc.num_size[b]};
-        launch_numeric_bin<V>(b, A, B, rl, pc.crpt64, ccol, cval, pc.ls, pc.spill, pc.hash, ps);
+        launch_numeric_bin<V>(b, A, B, rl, pc.crpt64, ccol, cval, pc.ls, pc.spill, pc.hash, ps, pc.num_shape[b]);
     });
 }
 
@@ -866,7 +869,7 @@ void exec_sequence(ops_exec *ex, hipStream_t s, bool measure = false)
         CK(hipGetLastError());
         dispatch_device_sized(ws, s, ex->busy_sym, [&](int b, hipStream_t ps) {
             launch_symbolic_bin(b, A, B, RowList{ex->bins.p, M, st->sym_size, b, st->next_sym + b}, ex->row_nnz.p,
-                                ws->short_hash, ws->sym_bitmap, SymSave{}, ps);
+                                ws->short_hash, ws->sym_bitmap, SymSave{}, ps, ws->sym_shape[b]);
         });
     }
     launch_scan(ex->row_nnz.p, ex->row_flop.p, A.rpt, M, ex->block_sums.p, ex->crpt.p, nullptr, st, bb, s);
@@ -887,10 +890,10 @@ void exec_sequence(ops_exec *ex, hipStream_t s, bool measure = false)
         const RowList rl{ex->bins.p, M, st->num_size, b, st->next_num + b};
         if (f64)
             launch_numeric_bin<double>(b, A, B, rl, ex->crpt.p, ex->ccol.p, ex->cval.p, ls, ex->spill.p,
-                                       ws->short_hash, ps);
+                                       ws->short_hash, ps, ws->num_shape[b]);
         else
             launch_numeric_bin<float>(b, A, B, rl, ex->crpt.p, ex->ccol.p, reinterpret_cast<float *>(ex->cval.p), ls,
-                                      ex->spill.p, ws->short_hash, ps);
+                                      ex->spill.p, ws->short_hash, ps, ws->num_shape[b]);
     });
 }
 
@@ -1109,6 +1112,7 @@ int ops_tuning_default(ops_tuning *t)
     }
     t->short_hash = kHashLin;
     t->sym_bitmap = 0;
+    for (int i = 0; i < kNumBins - 1; i++) t->sym_shape[i] = t->num_shape[i] = 0;
     return OPS_OK;
 }
 
@@ -1121,6 +1125,7 @@ int ops_tuning_caps(ops_tuning *c)
     }
     c->short_hash = kHashFib;
     c->sym_bitmap = 1;
+    for (int i = 0; i < kNumBins - 1; i++) c->sym_shape[i] = c->num_shape[i] = i < 5 ? kShapes - 1 : 0;
     return OPS_OK;
 }
 
@@ -1134,6 +1139,9 @@ int ops_workspace_set_tuning(ops_workspace *ws, const ops_tuning *t)
         if (t->num_bound[i] < 0 || t->num_bound[i] > kNumCap[i]) return OPS_INVALID;
         if (i > 0 && (t->sym_bound[i] < t->sym_bound[i - 1] || t->num_bound[i] < t->num_bound[i - 1]))
             return OPS_INVALID;
+        const int top = i < 5 ? kShapes - 1 : 0;
+        if (t->sym_shape[i] < 0 || t->sym_shape[i] > top || t->num_shape[i] < 0 || t->num_shape[i] > top)
+            return OPS_INVALID;
     }
     for (int i = 0; i < kNumBins - 1; i++) {
         ws->bounds.sym[i] = t->sym_bound[i];
@@ -1141,6 +1149,10 @@ int ops_workspace_set_tuning(ops_workspace *ws, const ops_tuning *t)
     }
     ws->short_hash = t->short_hash;
     ws->sym_bitmap = t->sym_bitmap != 0;
+    for (int i = 0; i < kNumBins - 1; i++) {
+        ws->sym_shape[i] = t->sym_shape[i];
+        ws->num_shape[i] = t->num_shape[i];
+    }
     return OPS_OK;
 }
 
@@ -1153,6 +1165,10 @@ int ops_workspace_get_tuning(const ops_workspace *ws, ops_tuning *t)
     }
     t->short_hash = ws->short_hash;
     t->sym_bitmap = ws->sym_bitmap ? 1 : 0;
+    for (int i = 0; i < kNumBins - 1; i++) {
+        t->sym_shape[i] = ws->sym_shape[i];
+        t->num_shape[i] = ws->num_shape[i];
+    }
     return OPS_OK;
 }
 

@@ -150,5 +150,33 @@ def test_every_tuning_candidate_is_exact(gpu):
     bad["sym_bound"][0] = 10 ** 6  # above the bin-0 table capacity
     with pytest.raises(Exception):
         ws.set_tuning(bad)
-    res = autotune.autotune(dA, dA, ws=ws, use_cache=False, reps=1)
+    res = autotune.autotune(dA, dA, ws=ws, use_cache=False, reps=1, shapes=False)
     assert res["tuning"] in autotune.candidates() and len(res["trials"]) == len(autotune.candidates())
+    # with the kernel-shape descent the pick is still exact
+    res = autotune.autotune(dA, dA, ws=ws, use_cache=False, reps=1)
+    assert len(res["trials"]) > len(autotune.candidates()) and ws.get_tuning() == res["tuning"]
+    rpt, col, val = spgemm(dA, dA, ws=ws).to_host()
+    oracle.compare(rpt, col, val, ref, 1e-12)
+
+
+@pytest.mark.parametrize("shape", [1, 2])
+def test_every_kernel_shape_is_exact(gpu, shape):
+    """Every alternative kernel shape (table size, lanes per row, block
+    size) of bins 0-4, symbolic and numeric, FP64 and FP32, with the
+    bitmap counting on and off."""
+    from opsparse_amd.spgemm import Workspace, default_tuning
+    mats = [G.rmat(12, 10, 13), G.stencil27(14, 2)]
+    ws = Workspace()
+    for bitmap in (0, 1):
+        t = default_tuning()
+        t["sym_shape"] = [shape] * 5 + [0, 0]
+        t["num_shape"] = [shape] * 5 + [0, 0]
+        t["sym_bitmap"] = bitmap
+        ws.set_tuning(t)
+        assert ws.get_tuning() == t
+        for A in mats:
+            for dt, npdt, tol in ((torch.float64, np.float64, 1e-12), (torch.float32, np.float32, 1e-5)):
+                dA = DeviceCSR.from_host(A, dt)
+                rpt, col, val = spgemm(dA, dA, ws=ws).to_host()
+                Ar = A.astype(npdt) if npdt == np.float32 else A
+                oracle.compare(rpt, col, val, oracle.spgemm(Ar, Ar), tol)
