@@ -26,7 +26,9 @@ Prints ONE JSON line.  value = 2 * products / time in GFLOP/s, time = the
 mean of the timed calls, each bracketed by synchronisations.  roofline = the
 kernel with the largest mean HIP-event time over three diagnostic calls made
 after the timed ones (events recorded on the pool stream it runs on; the
-timed calls record no events), with its algorithmic bytes per launch:
+timed calls record no events), with its algorithmic bytes per launch
+(frac = those bytes / event time / 8 TB/s; traffic_frac = the measured HBM
+bytes of the committed counter summary / event time / 8 TB/s):
   numeric bin  : (4+w)*P + (4+w)*Z + (12+w)*nnzA + 28*rows
   symbolic bin : 4*P + 12*nnzA + 16*rows
 (P products, Z output entries, nnzA A-nonzeros and rows of that bin; w the
@@ -109,6 +111,10 @@ def roofline_of(timings, w, workload_tag):
         for ln in open(tf):
             if ln.startswith("traffic_bytes_per_launch:"):
                 roof["traffic"] = int(float(ln.split(":")[1]))
+    if roof["traffic"]:
+        # bandwidth the kernel actually drew from HBM (measured bytes, not the
+        # no-reuse model): below `frac` when B's rows are served from caches
+        roof["traffic_frac"] = round(roof["traffic"] / (ms_k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     return roof
 
 
