@@ -11,12 +11,15 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <sstream>
 #include <stdexcept>
 #include <vector>
 
+#include "internal.hpp"
 #include "opsparse/core_algo/spgemm/spgemm.hpp"
 #include "outmem.hpp"
 #include "opsparse/io/matrix_market.hpp"
@@ -246,13 +249,42 @@ void CSR::construct(const std::string &mtx_file)
 
 // ---------------- SpgemmMeta ----------------
 
+SpgemmMeta::SpgemmMeta(CSR &C) { allocate_rpt(C); }
+
 SpgemmMeta::~SpgemmMeta() { release(); }
+
+void SpgemmMeta::allocate_rpt(CSR &C)
+{
+    OPSPARSE_CHECK_HIP(hipMalloc(&C.d_rpt, (static_cast<size_t>(C.M) + 1) * sizeof(mint)));
+}
+
+// Mirror layout (host and device): bin_size[kNumBin], max_row_nnz,
+// total_nnz, bin_offset[kNumBin].
+constexpr int kMirror = 2 * kNumBin + 2;
 
 void SpgemmMeta::allocate(CSR &C)
 {
     M = C.M;
     N = C.N;
     (void)workspace();
+    if (!combined_mem) {
+        combined_mem = static_cast<mint *>(std::calloc(kMirror, sizeof(mint)));
+        if (!combined_mem) throw std::bad_alloc();
+        OPSPARSE_CHECK_HIP(hipMalloc(&d_combined_mem, kMirror * sizeof(mint)));
+        OPSPARSE_CHECK_HIP(hipMemset(d_combined_mem, 0, kMirror * sizeof(mint)));
+        bin_size = combined_mem;
+        max_row_nnz = bin_size + kNumBin;
+        total_nnz = bin_size + kNumBin + 1;
+        bin_offset = bin_size + kNumBin + 2;
+        d_bin_size = d_combined_mem;
+        d_max_row_nnz = d_bin_size + kNumBin;
+        d_total_nnz = d_bin_size + kNumBin + 1;
+        d_bin_offset = d_bin_size + kNumBin + 2;
+    }
+    if (!stream) {
+        for (int i = 0; i < kNumBin; i++) OPSPARSE_CHECK_HIP(hipStreamCreateWithFlags(&pool_[i], hipStreamNonBlocking));
+        stream = pool_;
+    }
 }
 
 ops_workspace *SpgemmMeta::workspace()
@@ -268,6 +300,65 @@ void SpgemmMeta::release()
 {
     if (ws_) ops_workspace_destroy(ws_);
     ws_ = nullptr;
+    if (stream) {
+        for (int i = 0; i < kNumBin; i++) (void)hipStreamDestroy(pool_[i]);
+        stream = nullptr;
+    }
+    if (d_combined_mem) (void)hipFree(d_combined_mem);
+    std::free(combined_mem);
+    d_combined_mem = combined_mem = nullptr;
+    d_bin_size = d_bin_offset = d_max_row_nnz = d_total_nnz = d_bins = nullptr;
+    bin_size = bin_offset = max_row_nnz = total_nnz = nullptr;
+}
+
+hipStream_t SpgemmMeta::at(mint stream_idx) const
+{
+    return (stream && stream_idx >= 0 && stream_idx < kNumBin) ? stream[stream_idx] : nullptr;
+}
+
+void SpgemmMeta::memset_bin_size(mint stream_idx)
+{
+    OPSPARSE_CHECK_HIP(hipMemsetAsync(d_bin_size, 0, kNumBin * sizeof(mint), at(stream_idx)));
+}
+
+void SpgemmMeta::memset_all(mint stream_idx)
+{
+    OPSPARSE_CHECK_HIP(hipMemsetAsync(d_bin_size, 0, (kNumBin + 2) * sizeof(mint), at(stream_idx)));
+}
+
+void SpgemmMeta::d2h_bin_size(mint stream_idx)
+{
+    OPSPARSE_CHECK_HIP(
+        hipMemcpyAsync(bin_size, d_bin_size, kNumBin * sizeof(mint), hipMemcpyDeviceToHost, at(stream_idx)));
+}
+
+void SpgemmMeta::d2h_all(mint stream_idx)
+{
+    OPSPARSE_CHECK_HIP(
+        hipMemcpyAsync(bin_size, d_bin_size, (kNumBin + 2) * sizeof(mint), hipMemcpyDeviceToHost, at(stream_idx)));
+}
+
+void SpgemmMeta::h2d_bin_offset(mint stream_idx)
+{
+    OPSPARSE_CHECK_HIP(
+        hipMemcpyAsync(d_bin_offset, bin_offset, kNumBin * sizeof(mint), hipMemcpyHostToDevice, at(stream_idx)));
+}
+
+void SpgemmMeta::update_from_last_call(const mint *numeric_rows, const mint *symbolic_rows, mint max_nnz,
+                                       long long nnz)
+{
+    mint off = 0;
+    for (int b = 0; b < kNumBin; b++) {
+        bin_size[b] = numeric_rows[b];
+        bin_offset[b] = off;
+        off += numeric_rows[b];
+        symbolic_bin_size[b] = symbolic_rows[b];
+    }
+    *max_row_nnz = max_nnz;
+    *total_nnz = nnz > INT32_MAX ? -1 : static_cast<mint>(nnz);
+    total_nnz64 = nnz;
+    d_bins = const_cast<mint *>(opsamd::workspace_bins(ws_));
+    OPSPARSE_CHECK_HIP(hipMemcpy(d_combined_mem, combined_mem, kMirror * sizeof(mint), hipMemcpyHostToDevice));
 }
 
 // ---------------- entry points ----------------
@@ -302,17 +393,18 @@ void fill_timing(SpgemmTimings &t, const ops_timings &o)
 
 void fill_meta(SpgemmMeta &meta, const ops_timings &o)
 {
+    mint num[kNumBin], sym[kNumBin];
     for (int b = 0; b < kNumBin; b++) {
-        meta.bin_size[b] = static_cast<mint>(o.sym_rows[b]);
-        meta.numeric_bin_size[b] = static_cast<mint>(o.num_rows[b]);
+        num[b] = static_cast<mint>(o.num_rows[b]);
+        sym[b] = static_cast<mint>(o.sym_rows[b]);
     }
-    meta.max_row_nnz = o.max_row_nnz;
-    meta.total_nnz = o.nnz_c;
+    meta.update_from_last_call(num, sym, o.max_row_nnz, o.nnz_c);
 }
 
 void run(const CSR &A, const CSR &B, CSR &C, SpgemmMeta &meta, SpgemmTimings &timing, bool separate)
 {
     const double t0 = fast_clock_time();
+    meta.allocate(C);
     ops_workspace *ws = meta.workspace();
     meta.M = A.M;
     meta.N = B.N;
@@ -359,6 +451,7 @@ SpgemmStatus opsparse_spgemm_safe(const CSR &A, const CSR &B, CSR &C, SpgemmMeta
     }
     ops_workspace *ws = nullptr;
     try {
+        meta.allocate(C);
         ws = meta.workspace();
     } catch (const std::exception &e) {
         perf.status = SpgemmStatus::cuda_error;

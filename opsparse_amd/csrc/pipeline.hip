@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "internal.hpp"
 #include "launch.hpp"
 #include "opsparse_amd.h"
 #include "outmem.hpp"
@@ -1008,6 +1009,10 @@ int exec_create(ops_workspace *ws, ops_dtype dt, int use_graph, ops_exec **out, 
 }
 
 }  // namespace
+
+namespace opsamd {
+const int *workspace_bins(const ops_workspace *ws) { return ws ? ws->bins.p : nullptr; }
+}  // namespace opsamd
 
 extern "C" {
 

@@ -49,6 +49,33 @@ int main(int argc, char **argv)
         std::fprintf(stderr, "opsparse_spgemm and opsparse_spgemm_safe differ\n");
         return 1;
     }
+    // The workspace's public mirrors, as the reference's SpgemmMeta has
+    // them after a call: numeric bin sizes and offsets, max row nnz and
+    // nnz(C) on the host and (after d2h_all) the same from the device.
+    mint rows = 0;
+    for (int b = 0; b < kNumBin; b++) {
+        if (meta.bin_offset[b] != rows) {
+            std::fprintf(stderr, "bin_offset[%d] = %d, expected %d\n", b, meta.bin_offset[b], rows);
+            return 1;
+        }
+        rows += meta.bin_size[b];
+    }
+    if (rows > A.M || *meta.total_nnz != C.nnz || *meta.max_row_nnz <= 0 || !meta.stream || !meta.d_bin_size) {
+        std::fprintf(stderr, "meta mirrors: rows %d of %d, total_nnz %d (C %d), max_row_nnz %d\n", rows, A.M,
+                     *meta.total_nnz, C.nnz, *meta.max_row_nnz);
+        return 1;
+    }
+    mint saved[kNumBin + 2];
+    for (int i = 0; i < kNumBin + 2; i++) saved[i] = meta.bin_size[i];
+    for (int i = 0; i < kNumBin + 2; i++) meta.bin_size[i] = -7;
+    meta.d2h_all(0);
+    OPSPARSE_CHECK_HIP(hipStreamSynchronize(meta.stream[0]));
+    for (int i = 0; i < kNumBin + 2; i++) {
+        if (meta.bin_size[i] != saved[i]) {
+            std::fprintf(stderr, "device mirror %d = %d, host %d\n", i, meta.bin_size[i], saved[i]);
+            return 1;
+        }
+    }
     std::printf("products %ld nnz(C) %d total %.6f s\n", flop, C.nnz, timing.total);
 
     FILE *f = std::fopen(argv[2], "wb");
