@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -980,6 +981,17 @@ void exec_release(ops_exec *ex)
     if (ex->stats) (void)hipFree(ex->stats);
     ex->pa.release();
     ex->pb.release();
+    // DevBuf frees nothing by itself (the allocation-free entry point points
+    // its buffers at static memory), so every owned buffer is released here.
+    for (DevBuf<int> *b : {&ex->a_rpt, &ex->a_col, &ex->b_rpt, &ex->b_col, &ex->row_flop, &ex->row_nnz, &ex->bins,
+                           &ex->spill, &ex->cursor, &ex->wcut, &ex->ccol})
+        b->release();
+    ex->a_val.release();
+    ex->b_val.release();
+    ex->cval.release();
+    ex->block_sums.release();
+    ex->info.release();
+    ex->crpt.release();
     delete ex;
 }
 

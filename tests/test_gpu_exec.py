@@ -176,3 +176,32 @@ def test_exec_batch_sized_at_creation(gpu):
             rpt, col, val = C.rows_to_host(int(rb[i]), int(rb[i + 1]))
             oracle.compare(rpt, col, val, oracle.spgemm(As[i], Bs[i]), TOL[np.float64])
     ex.destroy()
+
+
+def test_exec_destroy_returns_memory(gpu):
+    """An executor's buffers (scratch, C at its capacity, packed operands)
+    are freed by ops_exec_destroy: creating and destroying batches of them
+    leaves free device memory where it was."""
+    rng = np.random.default_rng(12)
+    A = G.rmat(12, 8, 5)
+    dA = DeviceCSR.from_host(A, torch.float64)
+    As = [random_csr(200, 300, rng.integers(1, 20, 200), rng) for _ in range(8)]
+    Bs = [random_csr(300, 250, rng.integers(1, 20, 300), rng) for _ in range(8)]
+    dAs = [DeviceCSR.from_host(X, torch.float64) for X in As]
+    dBs = [DeviceCSR.from_host(X, torch.float64) for X in Bs]
+    cap = 64 << 20  # 64M entries of C each: 768 MB
+    Executor(dA, dA, torch.float64, capacity=cap).destroy()  # first use loads kernels, creates streams
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    for _ in range(4):
+        ex = Executor(dA, dA, torch.float64, capacity=cap)
+        ex.launch()
+        assert ex.result()[0] == OPS_OK
+        ex.destroy()
+        exb = Executor.batch(dAs, dBs, torch.float64, capacity=cap)
+        exb.launch()
+        assert exb.result()[0] == OPS_OK
+        exb.destroy()
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    assert free0 - free1 < (256 << 20), (free0, free1)
