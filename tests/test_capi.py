@@ -125,6 +125,16 @@ def test_invalid_arguments_without_gpu():
     assert _lib.lib().ops_plan_create(None, C.byref(a), C.byref(b), None, None, None) == _lib.OPS_INVALID
     assert _lib.lib().ops_workspace_create(None) == _lib.OPS_INVALID
     assert _lib.lib().ops_csr_free(None) == _lib.OPS_INVALID
+    ex = C.c_void_p()
+    L = _lib.lib()
+    # executors: no workspace, no operands, a negative instance count, no handle
+    assert L.ops_exec_create(None, 0, C.byref(a), C.byref(b), 0, 1, C.byref(ex)) == _lib.OPS_INVALID
+    assert L.ops_exec_create_batch(None, 0, 1, C.byref(a), C.byref(b), 0, 1, None, C.byref(ex)) == _lib.OPS_INVALID
+    assert L.ops_exec_create_batch(None, 0, -1, None, None, 0, 1, None, C.byref(ex)) == _lib.OPS_INVALID
+    assert L.ops_exec_launch(None, None) == _lib.OPS_INVALID
+    assert L.ops_exec_result(None, None, None, None) == _lib.OPS_INVALID
+    assert L.ops_exec_destroy(None) == _lib.OPS_INVALID
+    assert not ex.value
 
 
 def test_host_flop_matches_oracle():
