@@ -7,8 +7,10 @@ entries, so C takes 64-bit row offsets (ops_spgemm64_f64).  A step is one
 call (setup, binning, symbolic, scan, allocation of C, numeric) with inputs
 resident in HBM; C is released outside the timed region, as the reference
 releases it outside its timed call (benchmarks/opsparse_profile.cu:24-39).
-The 27-point stencil on 100^3 (FP64, seed 1) is measured after it and kept
-as a nested record (config.stencil_100).
+The 27-point stencil on 100^3 (FP64, seed 1) and the 4096-instance batch
+(FP64: fused call, packing included, the whole call replayed from one
+hipGraph, the looped plain call) are measured after it and kept as nested
+records (config.stencil_100, config.batch_4096).
 
 Other workloads (--workload):
   stencil  : the stencil alone
@@ -63,7 +65,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="rmat20", choices=["stencil", "uniform", "rmat20", "batch", "backward"])
-    p.add_argument("--no-stencil", action="store_true", help="rmat20: skip the nested stencil record")
+    p.add_argument("--no-stencil", action="store_true", help="rmat20: skip the nested stencil and batch records")
     p.add_argument("--cpu-iters", type=int, default=6)
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle (timing and row checks)")
     p.add_argument("--no-check", action="store_true", help="skip output checks")
@@ -377,13 +379,16 @@ def batch_instances(dX, row_base, dtype):
     return out, local_col
 
 
-def run_batch(args):
+def run_batch(args, nested=False):
     """Independent C_i = A_i*B_i packed block-diagonally: one fused call
     (also with the packing of the instances inside the timed span), the
     whole call replayed from a hipGraph with device-side bin sizes
     (packing, product and the shift of each C_i to its own columns), and
     the replayed numeric hipGraph of a plan, against the plain call looped
-    over the instances (row-slice descriptors of the packed A)."""
+    over the instances (row-slice descriptors of the packed A).
+
+    nested: FP64 only, no CPU leg, and a compact record (for the default
+    run's config) instead of a JSON line."""
     import numpy as np
     import torch
     import opsparse_amd.gen as G
@@ -397,7 +402,8 @@ def run_batch(args):
     log(f"batch: {args.batch_count} instances, rows {b.A.M}, nnz(A) {b.A.nnz}, products {P}")
     ws = Workspace()
     res = {}
-    for dt, w, tag in ((torch.float64, 8, "f64"), (torch.float32, 4, "f32")):
+    kinds = ((torch.float64, 8, "f64"),) if nested else ((torch.float64, 8, "f64"), (torch.float32, 4, "f32"))
+    for dt, w, tag in kinds:
         npdt = np.float64 if dt == torch.float64 else np.float32
         Ah, Bh = b.A.astype(npdt), b.B.astype(npdt)
         dA = DeviceCSR.from_host(Ah, dt)
@@ -409,7 +415,7 @@ def run_batch(args):
         nnz_c = c_last.nnz
         log(f"batch {tag}: fused {t_fused * 1e3:.3f} ms, nnz(C) {nnz_c}")
         cpu, err = None, None
-        if not args.no_cpu_baseline:
+        if not args.no_cpu_baseline and not nested:
             r1 = int(b.row_base[n_cpu])
             cpu, err = cpu_leg(Ah, Bh, 0, r1, max(1, args.cpu_iters // 2), f"instances 0..{n_cpu - 1}", c_last,
                                1e-12 if w == 8 else 1e-5, not args.no_check)
@@ -506,8 +512,23 @@ def run_batch(args):
                     "tuning": tune["tuning"], "cpu": cpu, "err": err}
         del dA, dB
         torch.cuda.empty_cache()
-    r, r32 = res["f64"], res["f32"]
+    r = res["f64"]
     roof = roofline_of(r["tims"], 8, "batch")
+    if nested:
+        return {"workload": f"{args.batch_count} independent C_i=A_i*B_i, n_i log-uniform in [64,16384], nnz/row "
+                            f"uniform in [4,32] per matrix, FP64, seed 3",
+                "products": int(P), "nnz_c": int(r["nnz_c"]), "gflops_fused": round(2.0 * P / r["t_fused"] / 1e9, 3),
+                "fused_ms": round(r["t_fused"] * 1e3, 3), "pack_and_fused_ms": round(r["t_packed_call"] * 1e3, 3),
+                "whole_call_graph_ms": round(r["t_exec"] * 1e3, 3),
+                "graph_numeric_replay_ms": round(r["t_graph"] * 1e3, 3),
+                "looped_plain_call_ms": round(r["t_loop"] * 1e3, 3),
+                "speedup_whole_call_graph_vs_loop": round(r["t_loop"] / r["t_exec"], 2),
+                "speedup_pack_and_fused_vs_loop": round(r["t_loop"] / r["t_packed_call"], 2),
+                "speedup_fused_vs_loop": round(r["t_loop"] / r["t_fused"], 2),
+                "tuning": r["tuning"], "roofline": roof,
+                "parity": {"whole_call_graph": "structure exact (own columns) on instances 0, middle and last",
+                           "max_norm_err": r["exec_err"]}}
+    r32 = res["f32"]
     cfg = {"workload": f"{args.batch_count} independent C_i=A_i*B_i, n_i log-uniform in [64,16384], nnz/row uniform "
                        f"in [4,32] per matrix, uniform distinct columns, U[-1,1], seed 3; packed block-diagonally "
                        f"and run as one fused, autotuned call (FP64 line; FP32 under fp32_*)",
@@ -654,6 +675,9 @@ def main():
             from opsparse_amd.spgemm import mem_cache_release
             mem_cache_release()
             cfg["stencil_100"] = nested_record(*run_single(args, "stencil"))
+            # the 4096-instance batch (FP64): fused, packing included, whole call from one hipGraph, looped
+            mem_cache_release()
+            cfg["batch_4096"] = run_batch(args, nested=True)
         out = line(gflops, args.steps, args.warmup, ms, "f64", cfg, roof, cpu)
     elif args.workload == "batch":
         out = run_batch(args)
