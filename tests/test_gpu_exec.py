@@ -189,19 +189,25 @@ def test_exec_destroy_returns_memory(gpu):
     Bs = [random_csr(300, 250, rng.integers(1, 20, 300), rng) for _ in range(8)]
     dAs = [DeviceCSR.from_host(X, torch.float64) for X in As]
     dBs = [DeviceCSR.from_host(X, torch.float64) for X in Bs]
-    cap = 64 << 20  # 64M entries of C each: 768 MB
-    Executor(dA, dA, torch.float64, capacity=cap).destroy()  # first use loads kernels, creates streams
-    torch.cuda.synchronize()
-    free0, _ = torch.cuda.mem_get_info()
-    for _ in range(4):
-        ex = Executor(dA, dA, torch.float64, capacity=cap)
-        ex.launch()
-        assert ex.result()[0] == OPS_OK
-        ex.destroy()
-        exb = Executor.batch(dAs, dBs, torch.float64, capacity=cap)
-        exb.launch()
-        assert exb.result()[0] == OPS_OK
-        exb.destroy()
-    torch.cuda.synchronize()
-    free1, _ = torch.cuda.mem_get_info()
-    assert free0 - free1 < (256 << 20), (free0, free1)
+    cap = 64 << 20  # 64M entries of C each: 864 MiB with the buffers' 1/8 slack
+
+    def cycles(n):
+        for _ in range(n):
+            ex = Executor(dA, dA, torch.float64, capacity=cap)
+            ex.launch()
+            assert ex.result()[0] == OPS_OK
+            ex.destroy()
+            exb = Executor.batch(dAs, dBs, torch.float64, capacity=cap)
+            exb.launch()
+            assert exb.result()[0] == OPS_OK
+            exb.destroy()
+        torch.cuda.synchronize()
+        return torch.cuda.mem_get_info()[0]
+
+    # The HIP runtime may hold on to one freed block of its own after the
+    # first cycles (seen: exactly one C capacity), so growth is measured
+    # between later cycles: an executor whose buffers were not freed would
+    # take at least one C capacity (864 MiB) per cycle, 3.4 GiB over four.
+    free1 = cycles(2)
+    free2 = cycles(4)
+    assert free1 - free2 < (256 << 20), (free1, free2)
