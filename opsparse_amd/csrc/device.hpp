@@ -719,11 +719,11 @@ constexpr size_t kMaxLds = 160 * 1024;
 
 // Stages one batch from per-thread ranges (thread t holds A-nonzero e with B
 // range [bs, bs + len) and A value av; len = 0 for none): the non-empty
-// ranges in thread order with their chunk offsets.  Returns their number;
-// `chunks` gets the batch's chunk count.  Every thread of the block calls it;
-// it ends with a barrier.  The previous walk of `st` must be complete (a
-// barrier) before.
-template <int BS, typename V>
+// ranges in thread order with their chunk offsets (chunks of CH products).
+// Returns their number; `chunks` gets the batch's chunk count.  Every thread
+// of the block calls it; it ends with a barrier.  The previous walk of `st`
+// must be complete (a barrier) before.
+template <int BS, typename V, int CH = kWave>
 __device__ __forceinline__ int chunk_stage_ranges(int e, int bs, int len, V av, ChunkStage<BS, V> &st, int &chunks)
 {
     constexpr int NW = BS / kWave;
@@ -731,7 +731,7 @@ __device__ __forceinline__ int chunk_stage_ranges(int e, int bs, int len, V av, 
     const int lane = lane_id();
     const int wave = tid / kWave;
     const int f = len > 0 ? 1 : 0;
-    const int nch = (len + kWave - 1) / kWave;
+    const int nch = (len + CH - 1) / CH;
     const int ic = wave_incl_scan(nch);
     const int iff = wave_incl_scan(f);
     if (lane == kWave - 1) {
@@ -762,7 +762,7 @@ __device__ __forceinline__ int chunk_stage_ranges(int e, int bs, int len, V av, 
     return tf;
 }
 
-template <int BS, typename V, typename Range>
+template <int BS, typename V, typename Range, int CH = kWave>
 __device__ __forceinline__ int chunk_stage(int c, int a_end, const int *__restrict__ acol, const V *__restrict__ aval,
                                            const Range &range, ChunkStage<BS, V> &st, int &chunks)
 {
@@ -773,7 +773,7 @@ __device__ __forceinline__ int chunk_stage(int c, int a_end, const int *__restri
         range(e, acol[e], bs, len);
         if (aval) av = aval[e];
     }
-    return chunk_stage_ranges<BS, V>(e, bs, len, av, st, chunks);
+    return chunk_stage_ranges<BS, V, CH>(e, bs, len, av, st, chunks);
 }
 
 // Walks every product of a staged batch (n ranges, `chunks` chunks) with all

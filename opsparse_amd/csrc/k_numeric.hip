@@ -189,6 +189,173 @@ struct RankAcc {
     }
 };
 
+// Element i of an array addressed by a 32-bit byte offset from a base held in
+// scalar registers (global_load with saddr): one shift per load instead of
+// the sign extension and 64-bit add of a pointer index.  The caller
+// guarantees i * sizeof(T) < 2^32.
+template <typename T>
+__device__ __forceinline__ T load_off32(const T *__restrict__ base, unsigned i)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * static_cast<unsigned>(sizeof(T)));
+}
+
+// The bitmap-rank accumulate walk of a staged batch: the same split of the
+// batch's products over the block's waves as block_walk (wave w takes one
+// contiguous slice, 64 consecutive products per step, NP steps gathered at
+// once in two ping-pong buffers), written for the fewest vector
+// instructions per step, which is what bounds this walk: every quantity
+// that is the same for the whole wave (slice bounds, the current range,
+// its B offset and A value) is held in scalar registers, so a step inside
+// one range costs one vector add for its B indices; B entries are read with
+// 32-bit offsets from scalar bases (OFF32: B holds fewer than 2^29
+// entries); and the rank of a product is one bit-field extract, one bit
+// count and one three-way add on the window word read from LDS.  Only a
+// step that crosses range starts finds each lane's range (the start mask
+// of block_walk).  Ends with a barrier.
+template <int BS, bool OFF32, typename V>
+__device__ __forceinline__ void rank_walk(const BlockStage<BS, V> &st, int n, int total, const RankAcc<V> &op)
+{
+    constexpr int NW = BS / kWave;
+    constexpr int NP = 4;
+    static_assert(BS <= kWave * kWave, "two-round owner search");
+    const int lane = lane_id();
+    const int wave = wave_uniform(static_cast<int>(threadIdx.x / kWave));
+    n = wave_uniform(n);
+    total = wave_uniform(total);
+    const int seg = (((total + NW - 1) / NW) + kWave - 1) & ~(kWave - 1);
+    const int s_beg = wave * seg;
+    const int s_end = min(total, s_beg + seg);
+    if (s_beg < s_end) {
+        const int g1 = (n + kWave - 1) / kWave;
+        int q = lane * g1;
+        int cnt = __popcll(__ballot(q < n && st.excl[q] <= s_beg));
+        const int lo = (cnt - 1) * g1;
+        q = lo + lane;
+        cnt = __popcll(__ballot(lane < g1 && q < n && st.excl[q] <= s_beg));
+        int qa = wave_uniform(lo + cnt - 1);
+        const unsigned long long below = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
+        // range qa in scalar registers: where the next range starts, its B
+        // index offset (B index = product position + offset) and A value
+        int q_next = 0, q_off = 0;
+        Acc q_av = Acc(0);
+        auto load_q = [&]() {
+            q_next = wave_uniform(qa + 1 <= n ? st.excl[qa + 1] : INT_MAX);
+            if (qa < n) {
+                const typename BlockStage<BS, V>::Ent en = st.ent[qa];
+                q_off = wave_uniform(en.off);
+                q_av = wave_uniform(static_cast<Acc>(en.av));
+            }
+        };
+        load_q();
+        // lanes past the slice's end read the batch's first product (a
+        // valid entry inside the window) and add nothing
+        const int safe = wave_uniform(st.ent[0].off);
+        const int w0 = op.w0;
+        const unsigned rbias = static_cast<unsigned>(-op.r0);
+        const unsigned nr = op.nr;
+        auto gather = [&](int b0, int (&bi)[NP], Acc (&aa)[NP], bool (&ok)[NP]) {
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                const int sb = b0 + s * kWave;
+                ok[s] = false;
+                bi[s] = safe;
+                aa[s] = Acc(0);
+                if (sb < s_end) {
+                    const int p = sb + lane;
+                    ok[s] = p < s_end;
+                    if (sb + kWave <= q_next) {
+                        bi[s] = ok[s] ? p + q_off : safe;
+                        aa[s] = q_av;
+                        if (sb + kWave == q_next) {
+                            qa++;
+                            load_q();
+                        }
+                    } else {
+                        const int qc = qa + 1 + lane;
+                        const int pos = qc <= n ? st.excl[qc] - sb : kWave + 1;
+                        const unsigned lo32 = pos > 0 && pos < 32 ? 1u << pos : 0u;
+                        const unsigned hi32 = pos >= 32 && pos < kWave ? 1u << (pos - 32) : 0u;
+                        const unsigned long long m =
+                            (static_cast<unsigned long long>(wave_or(hi32)) << 32) | wave_or(lo32);
+                        const int qo = qa + __popcll(m & below);
+                        qa = wave_uniform(static_cast<int>(qa + __popcll(m) + (__ballot(pos == kWave) != 0ull ? 1 : 0)));
+                        const typename BlockStage<BS, V>::Ent en = st.ent[min(qo, n - 1)];
+                        bi[s] = ok[s] ? p + en.off : safe;
+                        aa[s] = static_cast<Acc>(en.av);
+                        load_q();
+                    }
+                }
+            }
+        };
+        auto load = [&](const int (&bi)[NP], int (&c)[NP], V (&b)[NP]) {
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                if constexpr (OFF32) {
+                    c[s] = load_off32(op.bcol, static_cast<unsigned>(bi[s]));
+                    b[s] = load_off32(op.bval, static_cast<unsigned>(bi[s]));
+                } else {
+                    c[s] = op.bcol[bi[s]];
+                    b[s] = op.bval[bi[s]];
+                }
+            }
+        };
+        auto apply = [&](const int (&c)[NP], const V (&b)[NP], const Acc (&aa)[NP], const bool (&ok)[NP]) {
+            uint2 wd[NP];
+            unsigned cc[NP];
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                cc[s] = static_cast<unsigned>(c[s] - w0);
+                wd[s] = op.br[cc[s] >> 5];
+            }
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                const unsigned at = __popc(__builtin_amdgcn_ubfe(wd[s].x, 0u, cc[s] & 31u)) + wd[s].y + rbias;
+                if (ok[s] && at < nr) atomicAdd(op.vals + at, aa[s] * static_cast<Acc>(b[s]));
+            }
+        };
+        constexpr int stride = NP * kWave;
+        int biA[NP], biB[NP], cA[NP], cB[NP];
+        Acc aaA[NP], aaB[NP];
+        V bA[NP], bB[NP];
+        bool okA[NP], okB[NP];
+#pragma unroll
+        for (int s = 0; s < NP; s++) {
+            okB[s] = false;
+            aaB[s] = Acc(0);
+            cB[s] = w0;
+            bB[s] = V(0);
+        }
+        for (int base = s_beg;; base += 2 * stride) {
+            gather(base, biA, aaA, okA);
+            load(biA, cA, bA);
+            apply(cB, bB, aaB, okB);
+            gather(base + stride, biB, aaB, okB);
+            load(biB, cB, bB);
+            apply(cA, bA, aaA, okA);
+            if (base + 2 * stride >= s_end) break;
+        }
+        apply(cB, bB, aaB, okB);
+    }
+    __syncthreads();
+}
+
+// Every product of A row segment [a_beg, a_end) through rank_walk, in
+// batches of BS A-nonzeros.  Returns true when the row fit in one batch:
+// `st` then still holds it for another walk (rank_walk with n and total).
+template <int BS, bool OFF32, typename V, typename Range>
+__device__ __forceinline__ bool rank_for_each_product(int a_beg, int a_end, const int *__restrict__ acol,
+                                                      const V *__restrict__ aval, const Range &range,
+                                                      const RankAcc<V> &op, BlockStage<BS, V> &st, int &n, int &total)
+{
+    n = 0;
+    total = 0;
+    for (int c = a_beg; c < a_end; c += BS) {
+        n = block_stage<BS, V>(c, a_end, acol, aval, range, st, total);
+        if (total > 0) rank_walk<BS, OFF32, V>(st, n, total, op);
+    }
+    return a_end - a_beg <= BS;
+}
+
 // Bins 5-6: one row per block of BS threads, at most T/2 entries per row.
 // Rows whose columns span fewer than 32*T columns take the bitmap-rank path
 // of the long-row kernel in one window (a bitmap word per 32 columns from
@@ -605,7 +772,7 @@ struct RowRange {
     }
 };
 
-template <typename V>
+template <typename V, bool OFF32>
 __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const V *__restrict__ aval, const int *__restrict__ brpt,
                                                       const int *__restrict__ bcol, const V *__restrict__ bval,
@@ -676,8 +843,10 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
         while (w0 < ncols) {
             const int wend = (ncols - w0 <= R::WB) ? ncols : w0 + R::WB;
             const bool last = wend >= ncols;
-            // 1. mark
-            drain_vmem();
+            // 1. mark.  A kept row has no mark walk, so the previous window's
+            // output stores are left to complete during the rank scan (the
+            // drain before the accumulate walk then rarely waits).
+            if (!sbm) drain_vmem();
             int nxt = INT_MAX;
             int sn = 0, stot = 0;
             bool one = false;
@@ -696,9 +865,11 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                                                                     stot);
                 }
             }
-            nxt = unit_min<kWave>(nxt);
-            if (lane_id() == 0) s_min[wave] = nxt;
-            __syncthreads();
+            if (!sbm) {
+                nxt = unit_min<kWave>(nxt);
+                if (lane_id() == 0) s_min[wave] = nxt;
+                __syncthreads();
+            }
             // 2. rank: wave w owns words [w * WPW, (w + 1) * WPW) and its
             // lanes take them 64 consecutive words at a time (conflict-free
             // LDS reads); ranks come from DPP scans of the words' bit counts.
@@ -735,8 +906,10 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 br[wave * WPW + i * kWave + lane].y = static_cast<unsigned>(run + inc[i] - cnt[i]);
                 run += rsum[i];
             }
-            nxt = INT_MAX;
-            for (int w = 0; w < NU; w++) nxt = min(nxt, s_min[w]);
+            if (!sbm) {
+                nxt = INT_MAX;
+                for (int w = 0; w < NU; w++) nxt = min(nxt, s_min[w]);
+            }
             __syncthreads();
             // 3 + 4, per sub-window [ca, cb) of ranks [r0, r1)
             int r0 = 0, ca = w0, sub = 0;
@@ -771,25 +944,25 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     }
                     pf_ok = true;
                 }
-                RankAcc<V> op{bcol, bval, br, vals, w0, r0, static_cast<unsigned>(r1 - r0)};
+                const RankAcc<V> op{bcol, bval, br, vals, w0, r0, static_cast<unsigned>(r1 - r0)};
                 if (tot <= 2 * R::VCAP) {
                     // At most two sub-windows: walk the window's products over
                     // the ranges step 1 recorded and skip the other sub-window's
                     // ranks, instead of locating a cut in every B row.
                     // A row that fit in one batch is still staged from step 1.
                     if (one) {
-                        if (stot > 0) block_walk<kLongBS, true, V>(s_bst, sn, stot, op);
+                        if (stot > 0) rank_walk<kLongBS, OFF32, V>(s_bst, sn, stot, op);
                     } else if (ls.wcut) {
                         const WinCutRange range{ls.wcut, bcol, cstride, wi, nullptr};
-                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     } else if (sbm) {
                         const RowRange range{brpt};
-                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     } else {
                         const StoredRange range{brpt, cs(p), cs(p ^ 1), first};
-                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     }
                 } else {
@@ -797,13 +970,13 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     if (ls.wcut) {
                         const CutCursorRange range{ls.wcut,    bcol, cstride,    wi, cz(sub - 1), cz(sub),
                                                    cb,         sub == 0, cb == wend};
-                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
+                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     } else {
                         int unused = INT_MAX;
                         const int *cin = sub == 0 ? cs(p) : cz(sub - 1);
                         const CursorRange range{brpt,       bcol, cin, cz(sub), cb, first && sub == 0,
                                                 last && cb == wend, &unused};
-                        block_for_each_product<kLongBS, true, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
+                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     }
                 }
                 __syncthreads();
@@ -840,12 +1013,20 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 ca = cb;
                 sub++;
             }
-            for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
-            __syncthreads();
+            // A kept row's rank scan overwrites every word, so its bitmap is
+            // cleared once, after its last window.
+            if (!sbm) {
+                for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
+                __syncthreads();
+            }
             out += tot;
             first = false;
             p ^= 1;
             w0 = nxt == INT_MAX ? ncols : (nxt / R::WB) * R::WB;
+        }
+        if (sbm) {
+            for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
+            __syncthreads();
         }
     }
 }
@@ -889,8 +1070,10 @@ void set_attrs()
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(block_lds<8192, 1024, V>()));
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_block<2048, 256, V>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(block_lds<2048, 256, V>()));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(RankWin<V>::lds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(RankWin<V>::lds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(RankWin<V>::lds));
 }
 
 }  // namespace
@@ -951,9 +1134,14 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
             break;
         case 6: launch_block<8192, 1024, V>(A, B, rl, crpt, ccol, cval, s, kNumCu); break;
         default:
-            k_num_rank<V><<<row_grid(rl, kNumCu), kLongBS, RankWin<V>::lds, s>>>(
-                A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, B.N, crpt,
-                ccol, cval, ls);
+            if (ls.off32)
+                k_num_rank<V, true><<<row_grid(rl, kNumCu), kLongBS, RankWin<V>::lds, s>>>(
+                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, B.N,
+                    crpt, ccol, cval, ls);
+            else
+                k_num_rank<V, false><<<row_grid(rl, kNumCu), kLongBS, RankWin<V>::lds, s>>>(
+                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, B.N,
+                    crpt, ccol, cval, ls);
             break;
     }
 }

@@ -53,21 +53,6 @@ struct SymBitmap {
     }
 };
 
-// Sets the column's bit with an OR that returns nothing (no product waits on
-// an LDS result); the window's count is the bitmap's popcount afterwards.
-struct BitmapInsert {
-    const int *__restrict__ bcol;
-    unsigned *bm;
-    int w0;
-    using Item = int;
-    __device__ __forceinline__ int load(int k) const { return bcol[k]; }
-    __device__ __forceinline__ void apply(int col, double, int)
-    {
-        const int c = col - w0;
-        atomicOr(&bm[c >> 5], 1u << (c & 31));
-    }
-};
-
 // G lanes per row (G <= 64: BS/G rows per block; G == BS > 64: one row per
 // block, G/64 waves share the table).
 template <int G, int T, int BS, int H, bool BITMAP>
@@ -156,6 +141,130 @@ __global__ __launch_bounds__(BS) void k_sym_hash(const int *__restrict__ arpt, c
 constexpr int kLongBS = 1024;
 constexpr int kBitmapWords = kSymLongWindow / 32;  // 128 KB
 
+// Products per lane in a step of the long-row bitmap walk.
+constexpr int kOrPer = 4;
+constexpr int kOrChunk = kOrPer * kWave;  // products per step (one chunk)
+
+// Bitmap walk of a staged batch for the long-row symbolic kernel.  A step is
+// one chunk of kOrChunk consecutive products of one B range (sorted
+// columns), and lane l takes the chunk's products kOrPer*l .. kOrPer*l +
+// kOrPer - 1: the bits of a lane's products that fall in one bitmap word are
+// ORed in registers and the word gets one LDS OR.  With one product per lane,
+// a dense stretch of columns sends the 64 lanes of a step to two or three
+// words, and the LDS serialises ORs to one address (on R-MAT 2^20, 52% of the
+// kernel's LDS cycles were such address conflicts); here one OR instruction
+// reaches at most kOrChunk / 32 + 1 words through 64 / (32 / kOrPer) lanes
+// each.  A lane past its range's end reads the range's last entry again, and
+// a step past the wave's slice reads the batch's first product: ORing a bit
+// that is set already changes nothing, so no product needs a validity flag.
+// The chunk's owner range is held in scalar registers (chunk_walk).  NP steps
+// are loaded before the previous ones are applied.  Ends with a barrier.
+template <int BS>
+__device__ __forceinline__ void or4_walk(const ChunkStage<BS, double> &st, int n, int chunks,
+                                         const int *__restrict__ bcol, unsigned *bm, int w0)
+{
+    constexpr int NW = BS / kWave;
+    constexpr int NP = 2;
+    static_assert(BS <= kWave * kWave, "two-round owner search");
+    const int lane = lane_id();
+    const int wave = wave_uniform(static_cast<int>(threadIdx.x / kWave));
+    chunks = wave_uniform(chunks);
+    n = wave_uniform(n);
+    const int per = (chunks + NW - 1) / NW;
+    const int j0 = wave * per;
+    const int j1 = min(chunks, j0 + per);
+    if (j0 < j1) {
+        const int g1 = (n + kWave - 1) / kWave;
+        int q = lane * g1;
+        int cnt = __popcll(__ballot(q < n && st.cex[q] <= j0));
+        const int lo = (cnt - 1) * g1;
+        q = lo + lane;
+        cnt = __popcll(__ballot(lane < g1 && q < n && st.cex[q] <= j0));
+        int qa = wave_uniform(lo + cnt - 1);
+        int c_lo = 0, c_hi = 0, r_bs = 0, r_last = 0;
+        auto load_q = [&]() {
+            c_lo = wave_uniform(st.cex[qa]);
+            c_hi = wave_uniform(st.cex[qa + 1]);
+            const typename ChunkStage<BS, double>::Ent en = st.ent[qa];
+            r_bs = wave_uniform(en.bs);
+            r_last = r_bs + wave_uniform(en.len) - 1;
+        };
+        load_q();
+        const int safe = wave_uniform(st.ent[0].bs);
+        auto gather = [&](int jb, int (&c)[NP][kOrPer]) {
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                const int jj = jb + s;
+                int i0 = safe, last = safe;
+                if (jj < j1) {
+                    if (jj == c_hi) {
+                        qa++;
+                        load_q();
+                    }
+                    i0 = r_bs + (jj - c_lo) * kOrChunk + kOrPer * lane;
+                    last = r_last;
+                }
+#pragma unroll
+                for (int k = 0; k < kOrPer; k++) c[s][k] = bcol[min(i0 + k, last)];
+            }
+        };
+        auto apply = [&](const int (&c)[NP][kOrPer]) {
+#pragma unroll
+            for (int s = 0; s < NP; s++) {
+                unsigned acc = 0u;
+#pragma unroll
+                for (int k = 0; k < kOrPer; k++) {
+                    const unsigned cc = static_cast<unsigned>(c[s][k] - w0);
+                    acc |= 1u << (cc & 31u);
+                    if (k + 1 < kOrPer) {
+                        const unsigned nx = static_cast<unsigned>(c[s][k + 1] - w0);
+                        if ((nx >> 5) != (cc >> 5)) {
+                            atomicOr(&bm[cc >> 5], acc);
+                            acc = 0u;
+                        }
+                    } else {
+                        atomicOr(&bm[cc >> 5], acc);
+                    }
+                }
+            }
+        };
+        int cA[NP][kOrPer], cB[NP][kOrPer];
+        bool haveB = false;
+        for (int jb = j0;; jb += 2 * NP) {
+            gather(jb, cA);
+            if (haveB) apply(cB);
+            if (jb + NP >= j1) {
+                apply(cA);
+                break;
+            }
+            gather(jb + NP, cB);
+            haveB = true;
+            apply(cA);
+            if (jb + 2 * NP >= j1) {
+                apply(cB);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Sets the bits of every product of A row segment [a_beg, a_end) in the
+// window's bitmap, in batches of BS A-nonzeros (chunk_stage with kOrChunk
+// chunks + or4_walk).
+template <int BS, typename Range>
+__device__ __forceinline__ void or4_for_each_product(int a_beg, int a_end, const int *__restrict__ acol,
+                                                     const Range &range, const int *__restrict__ bcol, unsigned *bm,
+                                                     int w0, ChunkStage<BS, double> &st)
+{
+    for (int c = a_beg; c < a_end; c += BS) {
+        int chunks = 0;
+        const int n = chunk_stage<BS, double, Range, kOrChunk>(c, a_end, acol, static_cast<const double *>(nullptr),
+                                                               range, st, chunks);
+        if (chunks > 0) or4_walk<BS>(st, n, chunks, bcol, bm, w0);
+    }
+}
+
 // B range of an A-nonzero restricted to columns [w0, wend).  For a single
 // window covering all columns this is the whole row.  Otherwise *next
 // (thread-local) takes the minimum over the calls of the first column at or
@@ -226,10 +335,7 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
             int next = INT_MAX;
             const WindowRange range{brpt, bcol, w0, wend, w0 == 0 && wend == ncols, &next};
             drain_vmem();
-            BitmapInsert op{bcol, bm, w0};
-            int sn, stot;
-            chunk_for_each_product<kLongBS, false, double>(a_beg, a_end, acol, static_cast<const double *>(nullptr),
-                                                           range, op, s_bst, sn, stot);
+            or4_for_each_product<kLongBS>(a_beg, a_end, acol, range, bcol, bm, w0, s_bst);
             __syncthreads();
             if (save.bm) {
                 // One window (the host keeps bitmaps only then): a row long

@@ -194,6 +194,9 @@ struct LongScratch {
     const unsigned *bm = nullptr;
     const int *bm_slot = nullptr;
     int bm_words = 0;
+    // B's entries are addressed by 32-bit byte offsets: every entry index of
+    // B (up to its rpt[M]) is below 2^29.
+    bool off32 = false;
 };
 
 // wcut[k * (nwin + 1) + w] = first entry of B row k with column >= w *

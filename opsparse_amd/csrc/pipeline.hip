@@ -479,8 +479,13 @@ int guarded(ops_workspace *ws, F &&f)
 // Column-index span of A's descriptor: the highest A-nonzero index + 1,
 // which is what per-A-nonzero scratch must cover (descriptors may be row
 // slices whose rpt does not start at 0).
+// B entry indices below this bound let the numeric long-row walk address
+// B by 32-bit byte offsets (index * 8 < 2^32).
+constexpr long long kOff32Entries = 1ll << 29;
+
 long long a_index_span(const ops_csr *A, hipStream_t s)
 {
+    // (also used for B: its entry indices end at the value returned)
     if (A->M == 0) return 0;
     int last = 0;
     CK(hipMemcpyAsync(&last, A->rpt + A->M, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -512,6 +517,7 @@ void numeric_scratch(PlanCore &pc, const ops_csr *A, const ops_csr *B, DevBuf<in
 {
     if (pc.num_size[kNumBins - 1] > 0) {
         pc.ls.span = static_cast<int>(a_index_span(A, s));
+        pc.ls.off32 = a_index_span(B, s) < kOff32Entries;
         cursor.ensure(4 * static_cast<size_t>(pc.ls.span) + 4);
         pc.ls.cursor = cursor.p;
         const int nwin = rank_windows(B->N);
