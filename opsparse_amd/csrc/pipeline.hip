@@ -826,6 +826,7 @@ struct ops_exec {
     DevBuf<long long> block_sums, info;
     DevStats *stats = nullptr;
     int span = 0, nwin = 0;
+    bool off32 = false;  // B's entry indices stay below kOff32Entries
     bool cut = false;
     // bins holding rows in the measuring pass (symbolic, numeric)
     int busy_sym[kNumBins] = {}, busy_num[kNumBins] = {};
@@ -888,6 +889,7 @@ void exec_sequence(ops_exec *ex, hipStream_t s, bool measure = false)
     LongScratch ls;
     ls.cursor = ex->cursor.p;
     ls.span = ex->span;
+    ls.off32 = ex->off32;
     if (ex->cut) {
         launch_window_cuts(B, ex->nwin, ex->wcut.p, s);
         CK(hipGetLastError());
@@ -921,6 +923,10 @@ void exec_scratch(ops_exec *ex, hipStream_t s)
     // For a batch the packed A's last entry index is its entry count; a
     // descriptor is read on the device (its rpt[M]).
     ex->span = ex->batch ? ex->A.nnz : static_cast<int>(a_index_span(&ex->A, s));
+    // A batch's packed B starts at entry 0; a given B keeps the entry range
+    // it has at creation (patterns may change, entry counts may not).
+    const long long b_end = ex->batch ? ex->B.nnz : std::max<long long>(ex->B.nnz, a_index_span(&ex->B, s));
+    ex->off32 = b_end < kOff32Entries;
     ex->cursor.ensure(4 * static_cast<size_t>(ex->span) + 4);
     ex->nwin = rank_windows(ex->B.N);
     ex->cut = cut_table_wanted(ex->B.M, ex->B.N);

@@ -426,12 +426,29 @@ def test_long_rows_multi_window_cut_table_and_gallop(gpu, monkeypatch):
     col = np.minimum(col, N - 1)
     B = G.HostCSR(K, N, B3.rpt.copy(), col.astype(np.int32), B3.val.copy())
     A = random_csr(5, K, np.r_[rng.integers(300, 420, 3), 1, 0], rng)
+    # Rows with a dense first window: B rows put half their entries in
+    # columns [0, 16384) and the rest anywhere in 3 * 2^17 columns, so C's
+    # long rows hold about 20K columns in window 0 (two value passes) and
+    # sparse windows beyond.
+    Kd, Nd = 900, 3 * (1 << 17)
+    dl = rng.integers(180, 240, Kd)
+    rows = []
+    for i in range(Kd):
+        h = int(dl[i]) // 2
+        c = np.union1d(rng.choice(16384, h, replace=False), rng.choice(Nd, int(dl[i]) - h, replace=False))
+        rows.append(c)
+    drpt = np.zeros(Kd + 1, dtype=np.int32)
+    drpt[1:] = np.cumsum([len(c) for c in rows])
+    Bd = G.HostCSR(Kd, Nd, drpt, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, int(drpt[-1])))
+    Ad = random_csr(4, Kd, np.r_[rng.integers(200, 260, 2), 30, 1], rng)
     for mode in ("1", "0"):
         monkeypatch.setenv("OPSAMD_CUT_TABLE", mode)
         log(f"multi-window cut table={mode}")
         run(A, B)
         run(A, B, np.float32)
         run(A, B, wide=True)
+        run(Ad, Bd)
+        run(Ad, Bd, np.float32)
 
 
 def test_block_bins_rank_and_hash_paths(gpu):
