@@ -162,6 +162,40 @@ def test_load_pair_and_submatrix_against_scipy(tmp_path):
     assert r.returncode == 3 and "out_of_range" in r.stdout
 
 
+@pytest.mark.skipif(not os.path.exists(LIB_PATH), reason="library not built")
+@pytest.mark.parametrize("symmetry,field", [("general", "real"), ("general", "pattern"), ("symmetric", "real"),
+                                            ("symmetric", "pattern"), ("skew-symmetric", "real"),
+                                            ("general", "integer")])
+def test_mtx_loader_formats_against_scipy(tmp_path, symmetry, field):
+    """CSR(path) (the reference's MatrixMarket loader, csr.cu:187-352) on each
+    banner the reference accepts: symmetric and skew-symmetric entries are
+    mirrored (negated for skew), pattern entries read as 1.0, integer values
+    as doubles; the result equals scipy's reading of the same file."""
+    exe = build_pair_driver(str(tmp_path))
+    rng = np.random.default_rng(8)
+    n = 40
+    S = random_sparse(n, n, 5, 9)
+    if field == "integer":
+        S.data = rng.integers(-9, 10, S.data.size).astype(np.float64)
+    if symmetry == "symmetric":
+        L = sp.tril(S, format="csr")
+        S = L + sp.tril(L, k=-1).T
+    elif symmetry == "skew-symmetric":
+        L = sp.tril(S, k=-1, format="csr")
+        S = L - L.T
+    if field == "pattern":
+        S = S.copy()
+        S.data[:] = 1.0
+    path = write_mtx(str(tmp_path / "m.mtx"), S, symmetry=symmetry, field=field)
+    head = open(path).readline().split()
+    assert head[3:5] == [field, symmetry], head
+    ref = scipy.io.mmread(path).tocsr()
+    out = str(tmp_path / "o.bin")
+    subprocess.run([exe, "sub", path, str(n), str(n), "0", "0", out], check=True)
+    (got,) = read_csrs(out, 1)
+    assert same(got, ref)
+
+
 def hub_matrix(n):
     """Row 0 and column 0 full: A*A has n*n products through entry (0,0)'s
     column and row, more than INT32_MAX for n >= 46341."""
