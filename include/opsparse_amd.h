@@ -251,7 +251,8 @@ int ops_plan_numeric_graph(ops_plan *plan, ops_dtype dt, const ops_csr *A, const
  * again at every launch) and each ops_exec_launch replays it.  The
  * operands' descriptors (pointers, M, N, nnz) are fixed at creation; their
  * row offsets, columns and values may change between launches (the entry
- * counts staying the same).  C lives in the executor: 64-bit row offsets
+ * counts staying the same, and B's entries within the index range
+ * [rpt[0], rpt[M]) they span at creation).  C lives in the executor: 64-bit row offsets
  * and room for `capacity` entries (capacity <= 0: exactly nnz(C) of the
  * operands at creation, measured by one untimed run).  The workspace lends
  * its streams and tuning (taken at creation) and must outlive the executor.
