@@ -228,22 +228,31 @@ void dispatch_bins(ops_workspace *ws, hipStream_t s, const int *size, bool timin
         return;
     }
     // The fork is recorded before the first launch, so the pool streams do
-    // not wait for the bin that runs on s itself.
+    // not wait for a bin that runs on s itself.
     if (used > 1) CK(hipEventRecord(ws->ev_fork, s));
-    // Largest bins first so the long kernels start earliest; the first one
-    // runs on the caller's stream (no fork or join for it).
+    // Largest bins first so the long kernels start earliest.  The first one
+    // runs on the caller's stream (no fork or join for it), except when the
+    // long-row bin has rows: its kernel then runs long enough that a fork
+    // costs nothing, and every bin takes a pool stream, the k-th used bin
+    // pool[k].  The pool's streams are created back to back and the runtime
+    // gives streams hardware queues round-robin, so the largest bins then sit
+    // on different queues; the caller's stream may share a queue with the
+    // stream of another bin, which then starts only after the long-row kernel.
+    const bool all_pool = used > 1 && size[kNumBins - 1] > 0;
     const int order[kNumBins] = {7, 6, 5, 4, 3, 2, 1, 0};
     bool first = true;
+    int k = 0;
     for (int oi = 0; oi < kNumBins; oi++) {
         const int b = order[oi];
         if (size[b] <= 0) continue;
-        hipStream_t ps = first ? s : ws->pool[b];
-        if (!first) CK(hipStreamWaitEvent(ps, ws->ev_fork, 0));
+        const bool on_s = first && !all_pool;
+        hipStream_t ps = on_s ? s : ws->pool[all_pool ? k++ : b];
+        if (!on_s) CK(hipStreamWaitEvent(ps, ws->ev_fork, 0));
         if (timing) CK(hipEventRecord(ws->ev_k0[b], ps));
         launch(b, ps);
         CK(hipGetLastError());
         if (timing) CK(hipEventRecord(ws->ev_k1[b], ps));
-        if (!first) {
+        if (!on_s) {
             CK(hipEventRecord(ws->ev_join[b], ps));
             CK(hipStreamWaitEvent(s, ws->ev_join[b], 0));
         }

@@ -204,10 +204,12 @@ def test_exec_destroy_returns_memory(gpu):
         torch.cuda.synchronize()
         return torch.cuda.mem_get_info()[0]
 
-    # The HIP runtime may hold on to one freed block of its own after the
-    # first cycles (seen: exactly one C capacity), so growth is measured
-    # between later cycles: an executor whose buffers were not freed would
-    # take at least one C capacity (864 MiB) per cycle, 3.4 GiB over four.
+    # The HIP runtime may hold on to one freed block of its own (seen: exactly
+    # one C capacity, after the first cycles or between later ones), so growth
+    # is measured between later cycles and one C capacity is allowed: an
+    # executor whose buffers were not freed would take at least one C
+    # capacity (864 MiB) per cycle, 3.4 GiB over four.
+    c_bytes = cap * 12 * 9 // 8
     free1 = cycles(2)
     free2 = cycles(4)
-    assert free1 - free2 < (256 << 20), (free1, free2)
+    assert free1 - free2 < 2 * c_bytes, (free1, free2)
