@@ -960,10 +960,27 @@ struct WinCutRange {
     }
 };
 
+// First index in [lo, hi) whose column is >= bound (hi if none; lo if
+// lo >= hi): a binary search.
+__device__ __forceinline__ int lower_col(const int *__restrict__ bcol, int lo, int hi, int bound)
+{
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (bcol[m] < bound)
+            lo = m + 1;
+        else
+            hi = m;
+    }
+    return lo;
+}
+
 // Sub-window [.., cb) of column window wi: the range starts at the window's
 // cut (first sub-window) or where the previous sub-window stopped, and ends
-// at the first column >= cb, found by galloping no further than the
-// window's end cut.
+// at the first column >= cb.  With the unit cut table (ucut: the first
+// entry of every B row at each multiple of 2^kUnitCutShift columns, ustride
+// entries per row) that entry lies between the cuts of cb's unit, a short
+// binary search loaded alongside the window cut; without it, it is found by
+// galloping from the start, no further than the window's end cut.
 struct CutCursorRange {
     const int *__restrict__ wcut;
     const int *__restrict__ bcol;
@@ -974,12 +991,25 @@ struct CutCursorRange {
     int cb;
     bool first;
     bool to_end;  // cb is the window end
+    const int *__restrict__ ucut = nullptr;
+    int ustride = 0;
     __device__ __forceinline__ void operator()(int e, int k, int &bs, int &len) const
     {
         const int *c = wcut + static_cast<long long>(k) * stride;
         const int t = c[wi + 1];
         const int lo = first ? c[wi] : cur_in[e];
-        const int hi = to_end ? t : gallop_lower(bcol, lo, t, cb);
+        int hi;
+        if (to_end) {
+            hi = t;
+        } else if (ucut) {
+            // entries before the unit's cut are below cb, entries from the
+            // next unit's cut on are above it
+            const int *u = ucut + static_cast<long long>(k) * ustride + (cb >> kUnitCutShift);
+            const int l = max(lo, u[0]);
+            hi = min(t, lower_col(bcol, l, max(l, min(t, u[1])), cb));
+        } else {
+            hi = gallop_lower(bcol, lo, t, cb);
+        }
         cur_out[e] = hi;
         bs = lo;
         len = hi - lo;

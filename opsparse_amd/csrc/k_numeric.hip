@@ -968,8 +968,9 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 } else {
                     int n2, t2;
                     if (ls.wcut) {
-                        const CutCursorRange range{ls.wcut,    bcol, cstride,    wi, cz(sub - 1), cz(sub),
-                                                   cb,         sub == 0, cb == wend};
+                        const CutCursorRange range{ls.wcut,  bcol,     cstride,  wi,       cz(sub - 1),
+                                                   cz(sub),  cb,       sub == 0, cb == wend, ls.ucut,
+                                                   ls.nunit + 1};
                         rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     } else {
                         int unused = INT_MAX;
@@ -1154,7 +1155,7 @@ template void launch_numeric_bin<float>(int, const CsrIn &, const CsrIn &, const
 namespace {
 
 __global__ void k_window_cuts(const int *__restrict__ brpt, const int *__restrict__ bcol, int nb, int nwin,
-                              int *__restrict__ wcut)
+                              int shift, int *__restrict__ wcut)
 {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int stride = nwin + 1;
@@ -1168,7 +1169,7 @@ __global__ void k_window_cuts(const int *__restrict__ brpt, const int *__restric
     } else if (w == nwin) {
         v = hi;
     } else {
-        const int bound = w * kRankWindow;
+        const int bound = w << shift;
         int l = lo, h = hi;
         while (l < h) {
             const int m = (l + h) >> 1;
@@ -1184,12 +1185,12 @@ __global__ void k_window_cuts(const int *__restrict__ brpt, const int *__restric
 
 }  // namespace
 
-void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s)
+void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s, int shift)
 {
     const long long n = static_cast<long long>(B.M) * (nwin + 1);
     if (n <= 0) return;
     constexpr int BS = 256;
-    k_window_cuts<<<static_cast<unsigned>((n + BS - 1) / BS), BS, 0, s>>>(B.rpt, B.col, B.M, nwin, wcut);
+    k_window_cuts<<<static_cast<unsigned>((n + BS - 1) / BS), BS, 0, s>>>(B.rpt, B.col, B.M, nwin, shift, wcut);
 }
 
 }  // namespace opsamd

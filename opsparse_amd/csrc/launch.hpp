@@ -183,6 +183,8 @@ constexpr int kRankWindow = 1 << 17;
 constexpr long long kMaxCutBytes = 1ll << 30;
 
 // Scratch of the long-row bin.
+// Columns between the cuts of the unit cut table (LongScratch::ucut).
+constexpr int kUnitCutShift = 14;
 struct LongScratch {
     int *cursor = nullptr;      // four ints per A-nonzero index
     int span = 0;               // highest A-nonzero index + 1
@@ -197,11 +199,17 @@ struct LongScratch {
     // B's entries are addressed by 32-bit byte offsets: every entry index of
     // B (up to its rpt[M]) is below 2^29.
     bool off32 = false;
+    // With wcut, optionally the cut table at every 2^kUnitCutShift columns
+    // (nunit + 1 per B row), which bounds the search for a value
+    // sub-window's end in each B row.
+    const int *ucut = nullptr;
+    int nunit = 0;
 };
 
 // wcut[k * (nwin + 1) + w] = first entry of B row k with column >= w *
-// kRankWindow (w = 0: the row start, w = nwin: the row end).
-void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s);
+// kRankWindow (w = 0: the row start, w = nwin: the row end); with shift,
+// at every multiple of 2^shift columns instead.
+void launch_window_cuts(const CsrIn &B, int nwin, int *wcut, hipStream_t s, int shift = 17);
 
 // spill: scratch of (rows of bin kSpillBin + 2) ints (a count, a work
 // counter, the rows), or nullptr (block kernel only).

@@ -275,6 +275,15 @@ void read_kernel_ms(ops_workspace *ws, const int *size, double *ms)
 // Column windows of the long-row numeric kernel over B's columns.
 int rank_windows(int n) { return static_cast<int>((static_cast<long long>(n) + kRankWindow - 1) / kRankWindow); }
 
+// Whether the window-cut table comes with the unit cut table (sub-window
+// ends found between unit cuts).  OPSAMD_UNIT_CUTS=0 turns it off (galloping
+// from the sub-window start), for tests.
+bool unit_cuts_wanted()
+{
+    const char *env = std::getenv("OPSAMD_UNIT_CUTS");
+    return !(env && env[0] == '0');
+}
+
 // Whether the long-row numeric kernel gets a window-cut table of B: more
 // than one window and at most kMaxCutBytes.  OPSAMD_CUT_TABLE=0 turns the
 // table off (galloping path), for tests.
@@ -531,11 +540,23 @@ void numeric_scratch(PlanCore &pc, const ops_csr *A, const ops_csr *B, DevBuf<in
         pc.ls.cursor = cursor.p;
         const int nwin = rank_windows(B->N);
         if (allow_cut && cut_table_wanted(B->M, B->N)) {
-            wcut.ensure(static_cast<size_t>(B->M) * (nwin + 1));
+            // the unit cut table follows the window-cut table when it fits
+            // kMaxCutBytes too
+            const long long nunit = (static_cast<long long>(B->N) + (1 << kUnitCutShift) - 1) >> kUnitCutShift;
+            const size_t coarse = static_cast<size_t>(B->M) * (nwin + 1);
+            const size_t fine = static_cast<size_t>(B->M) * static_cast<size_t>(nunit + 1);
+            const bool units = static_cast<long long>(sizeof(int) * fine) <= kMaxCutBytes && unit_cuts_wanted();
+            wcut.ensure(coarse + (units ? fine : 0));
             launch_window_cuts(as_in(B), nwin, wcut.p, s);
             CK(hipGetLastError());
             pc.ls.wcut = wcut.p;
             pc.ls.nwin = nwin;
+            if (units) {
+                launch_window_cuts(as_in(B), static_cast<int>(nunit), wcut.p + coarse, s, kUnitCutShift);
+                CK(hipGetLastError());
+                pc.ls.ucut = wcut.p + coarse;
+                pc.ls.nunit = static_cast<int>(nunit);
+            }
         }
         // Kept bitmaps need a window's B ranges without a mark walk.
         if (!pc.ls.wcut && nwin > 1) {

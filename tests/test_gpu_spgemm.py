@@ -347,12 +347,15 @@ def test_output_cache_keeps_recent_blocks(gpu):
     mem_cache_release()
 
 
-def test_long_rows_many_rank_subwindows(gpu):
+@pytest.mark.parametrize("unit_cuts", ["1", "0"])
+def test_long_rows_many_rank_subwindows(gpu, monkeypatch, unit_cuts):
     """Rows of ~90K entries over two 2^17-column bitmap windows: the
     accumulate pass runs over several value sub-windows per window (12288
     entries each, FP64 accumulators for both value types), cut at the
-    column of each rank
-    boundary."""
+    column of each rank boundary; each sub-window's end in a B row is found
+    between the cuts of its 2^14-column unit, or (unit cut table off) by
+    galloping."""
+    monkeypatch.setenv("OPSAMD_UNIT_CUTS", unit_cuts)
     rng = np.random.default_rng(12)
     N = 200000
     A = random_csr(6, 1000, np.r_[rng.integers(350, 450, 4), rng.integers(1, 3, 2)], rng)
