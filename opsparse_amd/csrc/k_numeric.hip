@@ -718,11 +718,13 @@ constexpr int kLongBS = 1024;
 //  3. accumulate: the products are walked again and a*b is added to a
 //     compact LDS value array at that position.  When the window holds more
 //     than VCAP columns, this pass runs over column sub-windows of at most
-//     VCAP columns (cut at the column of rank r0 + VCAP).  With one or two
-//     sub-windows each walk reuses the B ranges step 1 found (still staged
-//     in LDS when the A row fits one batch) and skips the ranks of the other
-//     sub-window; with more, each sub-window locates its own B ranges with
-//     cursors, so no product is visited more than twice;
+//     VCAP columns (cut at the column of rank r0 + VCAP).  Each sub-window
+//     locates its own B ranges with cursors (the end in a B row searched
+//     between the cuts of the unit cut table when there is one), so every
+//     product is visited once; without the unit cut table, a window of two
+//     sub-windows instead walks twice over the B ranges step 1 found (still
+//     staged in LDS when the A row fits one batch), skipping the ranks of
+//     the other sub-window;
 //  4. write: values (already in rank order) and columns (placed by rank in
 //     LDS) are copied out contiguously.
 // No sort is needed and the value array only holds columns that exist, so a
@@ -945,10 +947,13 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     pf_ok = true;
                 }
                 const RankAcc<V> op{bcol, bval, br, vals, w0, r0, static_cast<unsigned>(r1 - r0)};
-                if (tot <= 2 * R::VCAP) {
-                    // At most two sub-windows: walk the window's products over
-                    // the ranges step 1 recorded and skip the other sub-window's
-                    // ranks, instead of locating a cut in every B row.
+                if (tot <= R::VCAP || (tot <= 2 * R::VCAP && !ls.ucut)) {
+                    // One sub-window, or two without the unit cut table: walk
+                    // the window's products over the ranges step 1 recorded
+                    // and skip the other sub-window's ranks, instead of
+                    // locating a cut in every B row by galloping.  (With the
+                    // unit cuts that search is short, and two sub-windows are
+                    // cheaper staged each on its own than walked twice.)
                     // A row that fit in one batch is still staged from step 1.
                     if (one) {
                         if (stot > 0) rank_walk<kLongBS, OFF32, V>(s_bst, sn, stot, op);

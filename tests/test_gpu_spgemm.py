@@ -414,8 +414,9 @@ def test_long_rows_multi_window_cut_table_and_gallop(gpu, monkeypatch):
     """Long rows whose columns lie in blocks 0, 2 and 5 of six 2^18-column
     blocks: the bitmap windows (2^17 columns) in between are empty and
     skipped, and the filled windows need several value sub-windows.  Run with
-    the window-cut table of B (default) and with the table turned off
-    (window boundaries found by galloping in each B row)."""
+    the window-cut table of B and its unit cuts (default), with the table
+    alone, and with the table turned off (window boundaries found by
+    galloping in each B row)."""
     rng = np.random.default_rng(14)
     W = 1 << 18
     N = 5 * W + 7
@@ -444,9 +445,13 @@ def test_long_rows_multi_window_cut_table_and_gallop(gpu, monkeypatch):
     drpt[1:] = np.cumsum([len(c) for c in rows])
     Bd = G.HostCSR(Kd, Nd, drpt, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, int(drpt[-1])))
     Ad = random_csr(4, Kd, np.r_[rng.integers(200, 260, 2), 30, 1], rng)
-    for mode in ("1", "0"):
+    # cut table with its unit cuts (windows of more than one value pass
+    # staged per sub-window), the cut table alone (two-pass windows walked
+    # twice over the same ranges), no table (galloping)
+    for mode, units in (("1", "1"), ("1", "0"), ("0", "1")):
         monkeypatch.setenv("OPSAMD_CUT_TABLE", mode)
-        log(f"multi-window cut table={mode}")
+        monkeypatch.setenv("OPSAMD_UNIT_CUTS", units)
+        log(f"multi-window cut table={mode} unit cuts={units}")
         run(A, B)
         run(A, B, np.float32)
         run(A, B, wide=True)
