@@ -822,19 +822,27 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
         const unsigned *const sbm = slot >= 0 ? ls.bm + static_cast<long long>(slot) * ls.bm_words : nullptr;
         long long out = crpt[r];
         const long long row_end = crpt[r + 1];
-        // First window: the one holding the smallest column of the row.
-        int mn = INT_MAX;
-        for (int e = a_beg + tid; e < a_end; e += kLongBS) {
-            const int k = acol[e];
-            if (brpt[k] < brpt[k + 1]) mn = min(mn, bcol[brpt[k]]);
+        // First window: the one holding the smallest column of the row.  A
+        // kept row starts at window 0 instead: its windows are visited in
+        // turn, an empty one costing only its rank scan, which is cheaper
+        // than the three dependent loads per A-nonzero and two barriers of
+        // the search (long rows of power-law inputs nearly all have columns
+        // in window 0).
+        int w0 = 0;
+        if (!sbm) {
+            int mn = INT_MAX;
+            for (int e = a_beg + tid; e < a_end; e += kLongBS) {
+                const int k = acol[e];
+                if (brpt[k] < brpt[k + 1]) mn = min(mn, bcol[brpt[k]]);
+            }
+            mn = unit_min<kWave>(mn);
+            if (lane_id() == 0) s_min[wave] = mn;
+            __syncthreads();
+            mn = INT_MAX;
+            for (int w = 0; w < NU; w++) mn = min(mn, s_min[w]);
+            __syncthreads();
+            w0 = mn == INT_MAX ? ncols : (mn / R::WB) * R::WB;
         }
-        mn = unit_min<kWave>(mn);
-        if (lane_id() == 0) s_min[wave] = mn;
-        __syncthreads();
-        mn = INT_MAX;
-        for (int w = 0; w < NU; w++) mn = min(mn, s_min[w]);
-        __syncthreads();
-        int w0 = mn == INT_MAX ? ncols : (mn / R::WB) * R::WB;
         bool first = true;
         int p = 0;
         // Kept bitmap words of the next window, loaded during this window's
