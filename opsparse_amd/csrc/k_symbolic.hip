@@ -337,7 +337,29 @@ __global__ __launch_bounds__(kLongBS) void k_sym_long(const int *__restrict__ ar
             drain_vmem();
             or4_for_each_product<kLongBS>(a_beg, a_end, acol, range, bcol, bm, w0, s_bst);
             __syncthreads();
-            if (save.bm) {
+            if (save.bm && save.by_index) {
+                // One window (the host keeps bitmaps only then), a slot for
+                // every row of the bin: the bitmap is copied out while it is
+                // counted and cleared, in one pass, and a row long enough
+                // for the numeric long-row kernel records its slot (a row
+                // below that leaves its copy unused).
+                unsigned *dst = save.bm + static_cast<size_t>(ridx) * nwords;
+                int c = 0;
+                for (int i = tid; i < nwords; i += kLongBS) {
+                    const unsigned x = bm[i];
+                    c += __popc(x);
+                    dst[i] = x;
+                    bm[i] = 0u;
+                }
+                const int tot = block_sum<kLongBS>(c, s_red);
+                if (tid == 0) {
+                    if (tot > save.min_nnz) {
+                        save.slot[r] = ridx;
+                        atomicAdd(save.count, 1);
+                    }
+                    cnt = tot;
+                }
+            } else if (save.bm) {
                 // One window (the host keeps bitmaps only then): a row long
                 // enough for the numeric long-row kernel leaves its bitmap
                 // in global memory, copied out while it is cleared.

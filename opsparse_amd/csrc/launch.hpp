@@ -158,6 +158,9 @@ struct SymSave {
     int *count = nullptr;
     int min_nnz = 0;
     int cap = 0;
+    // cap covers every row of the bin: a row's slot is its place in the
+    // bin's row list (no slot counter to wait for)
+    bool by_index = false;
 };
 // Computes row_nnz for the rows of one bin.
 // hash: family for the short-row bins 0-4 (kHashLin or kHashFib in device.hpp).

@@ -399,6 +399,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
             launch_fill_int(kb.slot.p, -1, M, s);
             CK(hipGetLastError());
             save = SymSave{kb.bm.p, kb.slot.p, &ws->d_stats->kept_rows, bb.num[kNumBins - 2], static_cast<int>(cap)};
+            save.by_index = cap >= long_rows;
             pc.ls.bm = kb.bm.p;
             pc.ls.bm_slot = kb.slot.p;
             pc.ls.bm_words = static_cast<int>(words);
