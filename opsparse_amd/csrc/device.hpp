@@ -911,10 +911,33 @@ __device__ __forceinline__ int gallop_lower(const int *__restrict__ bcol, int lo
     return l;
 }
 
+// First index in [lo, t) of a B row whose column is >= bound (t if none),
+// between the cuts of bound's unit in the unit cut table (u: the row's cuts
+// at bound's unit and the next): entries before the unit's cut are below
+// bound, entries from the next unit's cut on are above it, and a bound on a
+// unit boundary is the cut itself.
+__device__ __forceinline__ int unit_lower(const int *__restrict__ bcol, const int *__restrict__ u, int lo, int t,
+                                          int bound)
+{
+    const int l = max(lo, u[0]);
+    if ((bound & ((1 << kUnitCutShift) - 1)) == 0) return min(t, l);
+    int h = max(l, min(t, u[1]));
+    int x = l;
+    while (x < h) {
+        const int m = (x + h) >> 1;
+        if (bcol[m] < bound)
+            x = m + 1;
+        else
+            h = m;
+    }
+    return min(t, x);
+}
+
 // Cursor-carrying B range for window [w0, wend): lo is the first entry of
 // the B row not consumed by earlier windows; hi the first entry >= wend,
-// found by galloping from lo.  Tracks the smallest column beyond the window
-// so empty windows can be skipped.
+// found between the cuts of the unit cut table when one is given (ucut,
+// ustride ints per B row), else by galloping from lo.  With next_col, tracks
+// the smallest column beyond the window so empty windows can be skipped.
 struct CursorRange {
     const int *__restrict__ brpt;
     const int *__restrict__ bcol;
@@ -924,13 +947,21 @@ struct CursorRange {
     bool first;
     bool last;  // wend covers all columns
     int *next_col;
+    const int *__restrict__ ucut = nullptr;
+    int ustride = 0;
     __device__ __forceinline__ void operator()(int e, int k, int &bs, int &len) const
     {
         const int t = brpt[k + 1];
         const int lo = first ? brpt[k] : cur_in[e];
-        const int hi = last ? t : gallop_lower(bcol, lo, t, wend);
+        int hi;
+        if (last)
+            hi = t;
+        else if (ucut)
+            hi = unit_lower(bcol, ucut + static_cast<long long>(k) * ustride + (wend >> kUnitCutShift), lo, t, wend);
+        else
+            hi = gallop_lower(bcol, lo, t, wend);
         cur_out[e] = hi;
-        if (hi < t) *next_col = min(*next_col, bcol[hi]);
+        if (next_col && hi < t) *next_col = min(*next_col, bcol[hi]);
         bs = lo;
         len = hi - lo;
     }
@@ -960,20 +991,6 @@ struct WinCutRange {
     }
 };
 
-// First index in [lo, hi) whose column is >= bound (hi if none; lo if
-// lo >= hi): a binary search.
-__device__ __forceinline__ int lower_col(const int *__restrict__ bcol, int lo, int hi, int bound)
-{
-    while (lo < hi) {
-        const int m = (lo + hi) >> 1;
-        if (bcol[m] < bound)
-            lo = m + 1;
-        else
-            hi = m;
-    }
-    return lo;
-}
-
 // Sub-window [.., cb) of column window wi: the range starts at the window's
 // cut (first sub-window) or where the previous sub-window stopped, and ends
 // at the first column >= cb.  With the unit cut table (ucut: the first
@@ -1002,11 +1019,7 @@ struct CutCursorRange {
         if (to_end) {
             hi = t;
         } else if (ucut) {
-            // entries before the unit's cut are below cb, entries from the
-            // next unit's cut on are above it
-            const int *u = ucut + static_cast<long long>(k) * ustride + (cb >> kUnitCutShift);
-            const int l = max(lo, u[0]);
-            hi = min(t, lower_col(bcol, l, max(l, min(t, u[1])), cb));
+            hi = unit_lower(bcol, ucut + static_cast<long long>(k) * ustride + (cb >> kUnitCutShift), lo, t, cb);
         } else {
             hi = gallop_lower(bcol, lo, t, cb);
         }

@@ -202,8 +202,9 @@ struct BwdRank {
 // of the word popcounts ranks them (DPP inside each wave, wave totals
 // through LDS), and dC is staged by rank in sub-windows of VCAP entries.
 // Each sub-window walks its products with the whole block (block_stage +
-// block_walk over batches of kLongBS A-nonzeros; B ranges end at the
-// sub-window's last column, found by galloping from per-A-nonzero cursors);
+// block_walk over batches of kLongBS A-nonzeros; B ranges start at
+// per-A-nonzero cursors and end at the sub-window's last column, found
+// between the cuts of B's unit cut table, or by galloping without one);
 // after a batch, each thread adds its A-nonzero's dA slot to da.
 template <typename V>
 __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
@@ -213,7 +214,8 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
                                                       const long long *__restrict__ crpt,
                                                       const int *__restrict__ ccol, const V *__restrict__ dc,
                                                       V *__restrict__ da, V *__restrict__ db, int *__restrict__ cursor,
-                                                      int span, int a0, int b0)
+                                                      int span, int a0, int b0, const int *__restrict__ ucut,
+                                                      int ustride)
 {
     using R = BwdRankWin<V>;
     constexpr int NU = kLongBS / kWave;
@@ -281,9 +283,9 @@ __global__ __launch_bounds__(kLongBS) void k_bwd_rank(const int *__restrict__ ar
                 for (int i = tid; i < r1 - r0; i += kLongBS) gv[i] = dc[cpos + r0 + i];
                 __syncthreads();
                 drain_vmem();
-                int nxt = INT_MAX;
-                const CursorRange range{brpt, bcol, cursor + (parity ? span : 0), cursor + (parity ? 0 : span), cb,
-                                        first, cb >= ncols, &nxt};
+                const CursorRange range{brpt,  bcol,        cursor + (parity ? span : 0), cursor + (parity ? 0 : span),
+                                        cb,    first,       cb >= ncols,                  nullptr,
+                                        ucut,  ustride};
                 BwdRank<V> op{bcol, bval, br, gv, w0, r0, 0, db, b0, sda};
                 for (int c = a_beg; c < a_end; c += kLongBS) {
                     int total;
@@ -360,7 +362,7 @@ void backward_init_attributes()
 template <typename V>
 void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
                          const int *ccol, const V *dc, V *da, V *db, int a0, int b0, int *cursor, int span,
-                         hipStream_t s)
+                         const int *ucut, int nunit, hipStream_t s)
 {
     if (nrows <= 0) return;
     switch (bin) {
@@ -374,17 +376,17 @@ void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *row
         default:
             k_bwd_rank<V><<<nrows, kLongBS, BwdRankWin<V>::lds, s>>>(
                 A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rows, nrows,
-                B.N, crpt, ccol, dc, da, db, cursor, span, a0, b0);
+                B.N, crpt, ccol, dc, da, db, cursor, span, a0, b0, ucut, nunit + 1);
             break;
     }
 }
 
 template void launch_backward_bin<double>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *,
                                           const int *, const double *, double *, double *, int, int, int *, int,
-                                          hipStream_t);
+                                          const int *, int, hipStream_t);
 template void launch_backward_bin<float>(int, const CsrIn &, const CsrIn &, const int *, int, const long long *,
                                          const int *, const float *, float *, float *, int, int, int *, int,
-                                         hipStream_t);
+                                         const int *, int, hipStream_t);
 
 void launch_zero_values(void *p, long long n, bool f32, hipStream_t s)
 {

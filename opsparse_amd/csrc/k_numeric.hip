@@ -986,10 +986,9 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                                                    ls.nunit + 1};
                         rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     } else {
-                        int unused = INT_MAX;
                         const int *cin = sub == 0 ? cs(p) : cz(sub - 1);
                         const CursorRange range{brpt,       bcol, cin, cz(sub), cb, first && sub == 0,
-                                                last && cb == wend, &unused};
+                                                last && cb == wend, nullptr};
                         rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     }
                 }

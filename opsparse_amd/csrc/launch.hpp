@@ -225,11 +225,12 @@ void symbolic_init_attributes();
 
 // --- backward (k_backward.hip) ---
 // da[e - a0] receives the gradient of A entry e (a0 = A.rpt[0]), db[e - b0]
-// that of B entry e (b0 = B.rpt[0]).
+// that of B entry e (b0 = B.rpt[0]).  ucut: B's unit cut table (nunit + 1
+// ints per B row) for the long-row bin's sub-window ends, or nullptr.
 template <typename V>
 void launch_backward_bin(int bin, const CsrIn &A, const CsrIn &B, const int *rows, int nrows, const long long *crpt,
                          const int *ccol, const V *dc, V *da, V *db, int a0, int b0, int *cursor, int span,
-                         hipStream_t s);
+                         const int *ucut, int nunit, hipStream_t s);
 void backward_init_attributes();
 void launch_zero_values(void *p, long long n, bool f32, hipStream_t s);
 // row_nnz[i] = crpt[i+1] - crpt[i] (numeric binning key for the backward).

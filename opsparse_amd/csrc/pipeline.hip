@@ -705,15 +705,27 @@ void backward_impl(ops_workspace *ws, const ops_csr *A, const ops_csr *B, const 
     // per-A-nonzero cursors (two ints per A-nonzero index, double-buffered).
     int *cursor = nullptr;
     int span = 0;
+    const int *ucut = nullptr;
+    int nunit = 0;
     if (st.num_size[kNumBins - 1] > 0) {
         span = static_cast<int>(ea.end);
         ws->cursor.ensure(2 * static_cast<size_t>(span) + 2);
         cursor = ws->cursor.p;
+        // B's unit cut table bounds the search for each sub-window's end
+        const long long nu = (static_cast<long long>(B->N) + (1 << kUnitCutShift) - 1) >> kUnitCutShift;
+        const size_t fine = static_cast<size_t>(B->M) * static_cast<size_t>(nu + 1);
+        if (static_cast<long long>(sizeof(int) * fine) <= kMaxCutBytes && unit_cuts_wanted()) {
+            ws->wcut.ensure(fine);
+            launch_window_cuts(b, static_cast<int>(nu), ws->wcut.p, s, kUnitCutShift);
+            CK(hipGetLastError());
+            ucut = ws->wcut.p;
+            nunit = static_cast<int>(nu);
+        }
     }
     const int a0 = static_cast<int>(ea.begin), b0 = static_cast<int>(eb.begin);
     dispatch_bins(ws, s, st.num_size, false, [&](int bin, hipStream_t ps) {
         launch_backward_bin<V>(bin, a, b, rows ? rows + off[bin] : nullptr, st.num_size[bin], crpt, ccol, dc, da, db,
-                               a0, b0, cursor, span, ps);
+                               a0, b0, cursor, span, ucut, nunit, ps);
     });
 }
 
