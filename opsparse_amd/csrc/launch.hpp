@@ -185,9 +185,10 @@ constexpr int kRankWindow = 1 << 17;
 // locate window boundaries by galloping in each B row instead.
 constexpr long long kMaxCutBytes = 1ll << 30;
 
-// Scratch of the long-row bin.
 // Columns between the cuts of the unit cut table (LongScratch::ucut).
 constexpr int kUnitCutShift = 14;
+
+// Scratch of the long-row bin.
 struct LongScratch {
     int *cursor = nullptr;      // four ints per A-nonzero index
     int span = 0;               // highest A-nonzero index + 1
