@@ -19,6 +19,7 @@
 // in the sorted output), values accumulate at that rank in LDS, and the
 // window is written out in order (k_num_rank).
 #include <algorithm>
+#include <cstdlib>
 
 #include "device.hpp"
 #include "launch.hpp"
@@ -741,13 +742,19 @@ constexpr int kLongBS = 1024;
 // A-nonzeros' B rows and alternate between windows (the end found by
 // galloping), two alternate between the sub-windows of step 3 (with or
 // without the table).
-template <typename V>
+// Shapes: 1024-thread blocks with 12288 values per pass (one block per CU),
+// or 512-thread blocks with 4608 (two blocks per CU, for rows of at most
+// kLightRowMax entries, whose windows mostly fit one pass: two rows in
+// flight per CU hide each other's per-window waits).
+template <typename V, int BS = kLongBS>
 struct RankWin {
     static constexpr int WB = kRankWindow;                      // columns per bitmap window
     static constexpr int WORDS = WB / 32;                       // bitmap words (32 KB)
-    static constexpr int VCAP = 12288;                          // values per accumulate pass (96 KB)
+    static constexpr int VCAP = BS == kLongBS ? 12288 : 4608;   // values per accumulate pass (96 / 36 KB)
     static constexpr size_t lds = sizeof(Acc) * VCAP + sizeof(uint2) * WORDS;
 };
+constexpr int kLightBS = 512;
+constexpr long long kLightRowMax = 131072;
 
 // B range of an A-nonzero recorded by an earlier walk of the same window:
 // [start[e] (or the row start on the first window), end[e]).
@@ -774,27 +781,28 @@ struct RowRange {
     }
 };
 
-template <typename V, bool OFF32>
-__global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
+template <typename V, bool OFF32, int BS>
+__global__ __launch_bounds__(BS, 4) void k_num_rank(const int *__restrict__ arpt, const int *__restrict__ acol,
                                                       const V *__restrict__ aval, const int *__restrict__ brpt,
                                                       const int *__restrict__ bcol, const V *__restrict__ bval,
                                                       const RowList rl, int ncols, const long long *__restrict__ crpt,
-                                                      int *__restrict__ ccol, V *__restrict__ cval, LongScratch ls)
+                                                      int *__restrict__ ccol, V *__restrict__ cval, LongScratch ls,
+                                                      long long light_max)
 {
-    using R = RankWin<V>;
-    constexpr int NU = kLongBS / kWave;
-    constexpr int WPT = R::WORDS / kLongBS;  // bitmap words per thread
+    using R = RankWin<V, BS>;
+    constexpr int NU = BS / kWave;
+    constexpr int WPT = R::WORDS / BS;  // bitmap words per thread
     constexpr int WPW = R::WORDS / NU;       // bitmap words per wave in the rank scan
     constexpr int RPW = WPW / kWave;
-    static_assert(R::WORDS % kLongBS == 0, "words per thread");
+    static_assert(R::WORDS % BS == 0, "words per thread");
     extern __shared__ unsigned long long smem_u64[];
     Acc *vals = reinterpret_cast<Acc *>(smem_u64);
     uint2 *br = reinterpret_cast<uint2 *>(vals + R::VCAP);
-    __shared__ int s_scan[kLongBS / kWave + 1];
-    __shared__ int s_min[kLongBS / kWave];
+    __shared__ int s_scan[BS / kWave + 1];
+    __shared__ int s_min[BS / kWave];
     __shared__ int s_cut;
     __shared__ int s_tile;
-    __shared__ BlockStage<kLongBS, V> s_bst;
+    __shared__ BlockStage<BS, V> s_bst;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int lane = lane_id();
@@ -808,7 +816,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
 
     const int *rows;
     const int nrows = row_list(rl, rows);
-    for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
+    for (int i = 0; i < WPT; i++) br[tid + i * BS] = make_uint2(0u, 0u);
     __syncthreads();
 
     TileSched ts;
@@ -822,6 +830,10 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
         const unsigned *const sbm = slot >= 0 ? ls.bm + static_cast<long long>(slot) * ls.bm_words : nullptr;
         long long out = crpt[r];
         const long long row_end = crpt[r + 1];
+        // With light_max >= 0 the bin is split between two launches: the
+        // 512-thread shape takes the rows of at most light_max entries, the
+        // 1024-thread shape the others.
+        if (light_max >= 0 && ((row_end - out <= light_max) != (BS < kLongBS))) continue;
         // First window: the one holding the smallest column of the row.  A
         // kept row starts at window 0 instead: its windows are visited in
         // turn, an empty one costing only its rank scan, which is cheaper
@@ -831,7 +843,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
         int w0 = 0;
         if (!sbm) {
             int mn = INT_MAX;
-            for (int e = a_beg + tid; e < a_end; e += kLongBS) {
+            for (int e = a_beg + tid; e < a_end; e += BS) {
                 const int k = acol[e];
                 if (brpt[k] < brpt[k + 1]) mn = min(mn, bcol[brpt[k]]);
             }
@@ -867,11 +879,11 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 RankMark<V> op{bcol, br, w0};
                 if (ls.wcut) {
                     const WinCutRange range{ls.wcut, bcol, cstride, wi, &nxt};
-                    one = block_for_each_product<kLongBS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                    one = block_for_each_product<BS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                     stot);
                 } else {
                     const CursorRange range{brpt, bcol, cs(p), cs(p ^ 1), wend, first, last, &nxt};
-                    one = block_for_each_product<kLongBS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                    one = block_for_each_product<BS, false, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                     stot);
                 }
             }
@@ -942,7 +954,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     __syncthreads();
                     cb = s_cut;
                 }
-                for (int i = tid; i < r1 - r0; i += kLongBS) vals[i] = Acc(0);
+                for (int i = tid; i < r1 - r0; i += BS) vals[i] = Acc(0);
                 __syncthreads();
                 drain_vmem();
                 if (sbm && sub == 0 && !last) {
@@ -964,18 +976,18 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     // cheaper staged each on its own than walked twice.)
                     // A row that fit in one batch is still staged from step 1.
                     if (one) {
-                        if (stot > 0) rank_walk<kLongBS, OFF32, V>(s_bst, sn, stot, op);
+                        if (stot > 0) rank_walk<BS, OFF32, V>(s_bst, sn, stot, op);
                     } else if (ls.wcut) {
                         const WinCutRange range{ls.wcut, bcol, cstride, wi, nullptr};
-                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                        rank_for_each_product<BS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     } else if (sbm) {
                         const RowRange range{brpt};
-                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                        rank_for_each_product<BS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     } else {
                         const StoredRange range{brpt, cs(p), cs(p ^ 1), first};
-                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
+                        rank_for_each_product<BS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, sn,
                                                                  stot);
                     }
                 } else {
@@ -984,12 +996,12 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                         const CutCursorRange range{ls.wcut,  bcol,     cstride,  wi,       cz(sub - 1),
                                                    cz(sub),  cb,       sub == 0, cb == wend, ls.ucut,
                                                    ls.nunit + 1};
-                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
+                        rank_for_each_product<BS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     } else {
                         const int *cin = sub == 0 ? cs(p) : cz(sub - 1);
                         const CursorRange range{brpt,       bcol, cin, cz(sub), cb, first && sub == 0,
                                                 last && cb == wend, nullptr};
-                        rank_for_each_product<kLongBS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
+                        rank_for_each_product<BS, OFF32, V>(a_beg, a_end, acol, aval, range, op, s_bst, n2, t2);
                     }
                 }
                 __syncthreads();
@@ -1000,12 +1012,12 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                 // coalesced.
                 const int nsub = r1 - r0;
                 const long long lim = min(static_cast<long long>(nsub), row_end - (out + r0));
-                for (int i = tid; i < lim; i += kLongBS) cval[out + r0 + i] = static_cast<V>(vals[i]);
+                for (int i = tid; i < lim; i += BS) cval[out + r0 + i] = static_cast<V>(vals[i]);
                 __syncthreads();
                 int *scol = reinterpret_cast<int *>(vals);
                 const int wa = (ca - w0) >> 5;
                 const int we = (cb - w0 + 31) >> 5;
-                for (int w = wa + tid; w < we; w += kLongBS) {
+                for (int w = wa + tid; w < we; w += BS) {
                     const uint2 ew = br[w];
                     const unsigned all = ew.x;
                     unsigned bits = all;
@@ -1020,7 +1032,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
                     }
                 }
                 __syncthreads();
-                for (int i = tid; i < lim; i += kLongBS) ccol[out + r0 + i] = scol[i];
+                for (int i = tid; i < lim; i += BS) ccol[out + r0 + i] = scol[i];
                 __syncthreads();
                 r0 = r1;
                 ca = cb;
@@ -1029,7 +1041,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             // A kept row's rank scan overwrites every word, so its bitmap is
             // cleared once, after its last window.
             if (!sbm) {
-                for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
+                for (int i = 0; i < WPT; i++) br[tid + i * BS] = make_uint2(0u, 0u);
                 __syncthreads();
             }
             out += tot;
@@ -1038,7 +1050,7 @@ __global__ __launch_bounds__(kLongBS) void k_num_rank(const int *__restrict__ ar
             w0 = nxt == INT_MAX ? ncols : (nxt / R::WB) * R::WB;
         }
         if (sbm) {
-            for (int i = 0; i < WPT; i++) br[tid + i * kLongBS] = make_uint2(0u, 0u);
+            for (int i = 0; i < WPT; i++) br[tid + i * BS] = make_uint2(0u, 0u);
             __syncthreads();
         }
     }
@@ -1083,10 +1095,24 @@ void set_attrs()
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(block_lds<8192, 1024, V>()));
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_block<2048, 256, V>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(block_lds<2048, 256, V>()));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V, true>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V, true, kLongBS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(RankWin<V>::lds));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V, false>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V, false, kLongBS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(RankWin<V>::lds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V, true, kLightBS>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(RankWin<V, kLightBS>::lds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_num_rank<V, false, kLightBS>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(RankWin<V, kLightBS>::lds));
+}
+
+// Whether the long-row bin splits its rows between the two block shapes.
+// OPSAMD_LIGHT_ROWS=0 keeps every row in the 1024-thread shape, for tests.
+bool light_rows_wanted()
+{
+    const char *env = std::getenv("OPSAMD_LIGHT_ROWS");
+    return !(env && env[0] == '0');
 }
 
 }  // namespace
@@ -1147,15 +1173,31 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
             break;
         case 6: launch_block<8192, 1024, V>(A, B, rl, crpt, ccol, cval, s, kNumCu); break;
         default:
-            if (ls.off32)
-                k_num_rank<V, true><<<row_grid(rl, kNumCu), kLongBS, RankWin<V>::lds, s>>>(
+        {
+            // rows of at most kLightRowMax entries in 512-thread blocks, two
+            // per CU, then the others in 1024-thread blocks (each launch
+            // passes over the rows of the other one)
+            const long long lm = light_rows_wanted() ? kLightRowMax : -1;
+            const int grid = row_grid(rl, kNumCu);
+            if (ls.off32) {
+                if (lm >= 0)
+                    k_num_rank<V, true, kLightBS><<<grid, kLightBS, RankWin<V, kLightBS>::lds, s>>>(
+                        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl,
+                        B.N, crpt, ccol, cval, ls, lm);
+                k_num_rank<V, true, kLongBS><<<grid, kLongBS, RankWin<V>::lds, s>>>(
                     A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, B.N,
-                    crpt, ccol, cval, ls);
-            else
-                k_num_rank<V, false><<<row_grid(rl, kNumCu), kLongBS, RankWin<V>::lds, s>>>(
+                    crpt, ccol, cval, ls, lm);
+            } else {
+                if (lm >= 0)
+                    k_num_rank<V, false, kLightBS><<<grid, kLightBS, RankWin<V, kLightBS>::lds, s>>>(
+                        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl,
+                        B.N, crpt, ccol, cval, ls, lm);
+                k_num_rank<V, false, kLongBS><<<grid, kLongBS, RankWin<V>::lds, s>>>(
                     A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, B.N,
-                    crpt, ccol, cval, ls);
+                    crpt, ccol, cval, ls, lm);
+            }
             break;
+        }
     }
 }
 

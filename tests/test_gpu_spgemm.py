@@ -447,11 +447,13 @@ def test_long_rows_multi_window_cut_table_and_gallop(gpu, monkeypatch):
     Ad = random_csr(4, Kd, np.r_[rng.integers(200, 260, 2), 30, 1], rng)
     # cut table with its unit cuts (windows of more than one value pass
     # staged per sub-window), the cut table alone (two-pass windows walked
-    # twice over the same ranges), no table (galloping)
-    for mode, units in (("1", "1"), ("1", "0"), ("0", "1")):
+    # twice over the same ranges) with every row in the 1024-thread shape,
+    # no table (galloping)
+    for mode, units, light in (("1", "1", "1"), ("1", "0", "0"), ("0", "1", "1")):
         monkeypatch.setenv("OPSAMD_CUT_TABLE", mode)
         monkeypatch.setenv("OPSAMD_UNIT_CUTS", units)
-        log(f"multi-window cut table={mode} unit cuts={units}")
+        monkeypatch.setenv("OPSAMD_LIGHT_ROWS", light)
+        log(f"multi-window cut table={mode} unit cuts={units} light rows={light}")
         run(A, B)
         run(A, B, np.float32)
         run(A, B, wide=True)
