@@ -1107,12 +1107,23 @@ void set_attrs()
                               static_cast<int>(RankWin<V, kLightBS>::lds));
 }
 
-// Whether the long-row bin splits its rows between the two block shapes.
-// OPSAMD_LIGHT_ROWS=0 keeps every row in the 1024-thread shape, for tests.
-bool light_rows_wanted()
+// Whether the long-row bin splits its rows between its two block shapes
+// (two launches).  OPSAMD_LIGHT_ROWS=0 keeps every row in the 1024-thread
+// shape, for tests; a device-sized list whose work counter has no second
+// counter beside it keeps one launch too.
+bool split_rows(const RowList &rl)
 {
     const char *env = std::getenv("OPSAMD_LIGHT_ROWS");
-    return !(env && env[0] == '0');
+    return !(env && env[0] == '0') && !(rl.next && !rl.next2);
+}
+
+// The row list of the long-row bin's second launch: the same rows under the
+// second work counter.
+RowList second_launch(const RowList &rl)
+{
+    RowList r = rl;
+    r.next = rl.next2;
+    return r;
 }
 
 }  // namespace
@@ -1177,7 +1188,8 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
             // rows of at most kLightRowMax entries in 512-thread blocks, two
             // per CU, then the others in 1024-thread blocks (each launch
             // passes over the rows of the other one)
-            const long long lm = light_rows_wanted() ? kLightRowMax : -1;
+            const long long lm = split_rows(rl) ? kLightRowMax : -1;
+            const RowList rh = lm >= 0 ? second_launch(rl) : rl;
             const int grid = row_grid(rl, kNumCu);
             if (ls.off32) {
                 if (lm >= 0)
@@ -1185,7 +1197,7 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
                         A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl,
                         B.N, crpt, ccol, cval, ls, lm);
                 k_num_rank<V, true, kLongBS><<<grid, kLongBS, RankWin<V>::lds, s>>>(
-                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, B.N,
+                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rh, B.N,
                     crpt, ccol, cval, ls, lm);
             } else {
                 if (lm >= 0)
@@ -1193,7 +1205,7 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
                         A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl,
                         B.N, crpt, ccol, cval, ls, lm);
                 k_num_rank<V, false, kLongBS><<<grid, kLongBS, RankWin<V>::lds, s>>>(
-                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl, B.N,
+                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rh, B.N,
                     crpt, ccol, cval, ls, lm);
             }
             break;

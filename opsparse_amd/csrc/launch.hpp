@@ -77,6 +77,9 @@ struct DevStats {
     // work counters of device-sized launches (RowList::next), per bin
     int next_sym[kNumBins];
     int next_num[kNumBins];
+    // work counters of the second launch of a numeric bin that splits its
+    // rows between two block shapes (RowList::next2)
+    int next_num2[kNumBins];
 };
 
 struct CsrIn {
@@ -95,13 +98,17 @@ struct CsrIn {
 // tiles, so a launch recorded in a graph serves any bin sizes.  With
 // `next` (a counter zeroed before the launch), blocks take their tiles from
 // it one at a time instead of by a fixed stride, so a grid narrower than
-// the tile count stays balanced whatever the rows cost.
+// the tile count stays balanced whatever the rows cost.  A bin that runs
+// its rows in two launches (each taking the rows of one block shape and
+// passing over the others) gives the second launch `next2`, a counter of its
+// own; a device-sized list with `next` and no `next2` runs in one launch.
 struct RowList {
     const int *rows = nullptr;
     int n = 0;
     const int *dsize = nullptr;
     int bin = 0;
     int *next = nullptr;
+    int *next2 = nullptr;
 };
 
 // Grid of a per-bin kernel over tiles of `per_tile` rows.  Host-sized: one

@@ -940,7 +940,7 @@ void exec_sequence(ops_exec *ex, hipStream_t s, bool measure = false)
         ls.nwin = ex->nwin;
     }
     dispatch_device_sized(ws, s, ex->busy_num, [&](int b, hipStream_t ps) {
-        const RowList rl{ex->bins.p, M, st->num_size, b, st->next_num + b};
+        const RowList rl{ex->bins.p, M, st->num_size, b, st->next_num + b, st->next_num2 + b};
         if (f64)
             launch_numeric_bin<double>(b, A, B, rl, ex->crpt.p, ex->ccol.p, ex->cval.p, ls, ex->spill.p,
                                        ws->short_hash, ps, ws->num_shape[b]);

@@ -96,6 +96,36 @@ def test_exec_long_rows_spill_and_cut_table(gpu):
         ex.destroy()
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_exec_split_bins(gpu, graph):
+    """The long-row bin runs its rows in two launches of different block
+    shapes, each taking the rows of its shape under a work counter of its
+    own: a row of ~165K entries in the 1024-thread shape, rows of ~9K in the
+    512-thread one, beside rows of the block bins (~1.5K and ~3.6K
+    entries); replayed with new values, checked after every launch and
+    against the drop-in call."""
+    rng = np.random.default_rng(47)
+    K, N = 2000, 1 << 20
+    B = random_csr(K, N, rng.integers(250, 350, K), rng)
+    A = random_csr(8, K, np.array([600, 30, 40, 5, 6, 12, 2, 1]), rng)
+    dA, dB = DeviceCSR.from_host(A, torch.float64), DeviceCSR.from_host(B, torch.float64)
+    lens = np.diff(oracle.spgemm(A, B)[0])
+    assert lens[0] > 131072 and 4096 < lens[1] <= 131072 and 1024 < lens[3] <= 2048 < lens[5] <= 4096, lens
+    ex = Executor(dA, dB, torch.float64, graph=graph)
+    for step, scale in enumerate((1.0, 1.0, -0.5)):
+        A2 = G.HostCSR(A.M, A.N, A.rpt, A.col, A.val * scale)
+        refill(dA, A2)
+        ex.launch()
+        e = check_result(ex, A2, B)
+        log(f"step {step}: nnz {e['nnz']}, max_norm {e['max_norm']:.2e}")
+        C = spgemm(dA, dB)
+        rpt, col, val = C.to_host()
+        oracle.compare(rpt, col, val, oracle.spgemm(A2, B), TOL[np.float64])
+        C.free()
+        torch.cuda.synchronize()
+    ex.destroy()
+
+
 def test_exec_capacity_overflow_writes_nothing(gpu):
     A = G.rmat(11, 8, 3)
     dA = DeviceCSR.from_host(A, torch.float64)
