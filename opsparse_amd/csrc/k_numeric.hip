@@ -754,7 +754,6 @@ struct RankWin {
     static constexpr size_t lds = sizeof(Acc) * VCAP + sizeof(uint2) * WORDS;
 };
 constexpr int kLightBS = 512;
-constexpr long long kLightRowMax = 131072;
 
 // B range of an A-nonzero recorded by an earlier walk of the same window:
 // [start[e] (or the row start on the first window), end[e]).
@@ -1186,27 +1185,36 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
         default:
         {
             // rows of at most kLightRowMax entries in 512-thread blocks, two
-            // per CU, then the others in 1024-thread blocks (each launch
-            // passes over the rows of the other one)
+            // per CU, then the others in 1024-thread blocks: over the two
+            // parts of a host-sized list ordered by the numeric scatter,
+            // else each launch over every row, passing over the other's
             const long long lm = split_rows(rl) ? kLightRowMax : -1;
-            const RowList rh = lm >= 0 ? second_launch(rl) : rl;
-            const int grid = row_grid(rl, kNumCu);
+            RowList rlo = rl, rh = lm >= 0 ? second_launch(rl) : rl;
+            if (lm >= 0 && ls.nheavy >= 0 && rl.rows && !rl.dsize && !rl.next && ls.nheavy <= rl.n) {
+                // the list is ordered: rows of the 512-thread shape, then the others
+                rlo.n = rl.n - ls.nheavy;
+                rh.rows = rl.rows + rlo.n;
+                rh.n = ls.nheavy;
+            }
+            const int grid = row_grid(rlo, kNumCu), hgrid = row_grid(rh, kNumCu);
             if (ls.off32) {
-                if (lm >= 0)
+                if (lm >= 0 && rlo.n > 0)
                     k_num_rank<V, true, kLightBS><<<grid, kLightBS, RankWin<V, kLightBS>::lds, s>>>(
-                        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl,
+                        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rlo,
                         B.N, crpt, ccol, cval, ls, lm);
-                k_num_rank<V, true, kLongBS><<<grid, kLongBS, RankWin<V>::lds, s>>>(
-                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rh, B.N,
-                    crpt, ccol, cval, ls, lm);
+                if (rh.n > 0)
+                    k_num_rank<V, true, kLongBS><<<hgrid, kLongBS, RankWin<V>::lds, s>>>(
+                        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rh,
+                        B.N, crpt, ccol, cval, ls, lm);
             } else {
-                if (lm >= 0)
+                if (lm >= 0 && rlo.n > 0)
                     k_num_rank<V, false, kLightBS><<<grid, kLightBS, RankWin<V, kLightBS>::lds, s>>>(
-                        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rl,
+                        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rlo,
                         B.N, crpt, ccol, cval, ls, lm);
-                k_num_rank<V, false, kLongBS><<<grid, kLongBS, RankWin<V>::lds, s>>>(
-                    A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rh, B.N,
-                    crpt, ccol, cval, ls, lm);
+                if (rh.n > 0)
+                    k_num_rank<V, false, kLongBS><<<hgrid, kLongBS, RankWin<V>::lds, s>>>(
+                        A.rpt, A.col, static_cast<const V *>(A.val), B.rpt, B.col, static_cast<const V *>(B.val), rh,
+                        B.N, crpt, ccol, cval, ls, lm);
             }
             break;
         }

@@ -184,12 +184,15 @@ __global__ __launch_bounds__(kScatBS) void k_bin_scatter(const int *__restrict__
                                                          DevStats *__restrict__ st, int *__restrict__ bins,
                                                          const BinBounds bb)
 {
-    __shared__ int s_cnt[kNumBins];
-    __shared__ int s_base[kNumBins];
+    // classes: the bins, then (numeric) the long-row bin's rows of more
+    // than kLightRowMax entries, placed at the end of that bin's list
+    constexpr int kClasses = kNumBins + 1;
+    __shared__ int s_cnt[kClasses];
+    __shared__ int s_base[kClasses];
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
-    if (tid < kNumBins) s_cnt[tid] = 0;
+    if (tid < kClasses) s_cnt[tid] = 0;
     __syncthreads();
     const int r0 = blockIdx.x * (kScatBS * kScatItems);
     int bin[kScatItems], rank[kScatItems];
@@ -200,10 +203,10 @@ __global__ __launch_bounds__(kScatBS) void k_bin_scatter(const int *__restrict__
         rank[i] = 0;
         if (r < M) {
             const int v = key[r];
-            if (v > 0) bin[i] = which ? dev_num_bin(v, bb) : dev_sym_bin(v, bb);
+            if (v > 0) bin[i] = which ? (v > kLightRowMax ? kNumBins : dev_num_bin(v, bb)) : dev_sym_bin(v, bb);
         }
 #pragma unroll
-        for (int b = 0; b < kNumBins; b++) {
+        for (int b = 0; b < kClasses; b++) {
             const unsigned long long m = __ballot(bin[i] == b);
             if (!m) continue;
             const int leader = __ffsll(static_cast<long long>(m)) - 1;
@@ -220,6 +223,11 @@ __global__ __launch_bounds__(kScatBS) void k_bin_scatter(const int *__restrict__
         int off = 0;
         for (int i = 0; i < tid; i++) off += size[i];
         s_base[tid] = s_cnt[tid] ? off + atomicAdd(&fill[tid], s_cnt[tid]) : 0;
+    }
+    if (tid == kNumBins && s_cnt[tid]) {
+        int off = 0;
+        for (int i = 0; i < kNumBins; i++) off += st->num_size[i];
+        s_base[tid] = off - st->num_heavy + atomicAdd(&st->heavy_fill, s_cnt[tid]);
     }
     __syncthreads();
 #pragma unroll
@@ -259,6 +267,7 @@ __global__ __launch_bounds__(kScanBS) void k_scan_reduce(const int *__restrict__
             const int n = row_nnz[r];
             sum += static_cast<unsigned long long>(n);
             mx = max(mx, n);
+            if (n > kLightRowMax) atomicAdd(&st->num_heavy, 1);
             if (n > 0) {
                 const int b = dev_num_bin(n, bb);
                 atomicAdd(&s_hist[b], 1);

@@ -80,6 +80,10 @@ struct DevStats {
     // work counters of the second launch of a numeric bin that splits its
     // rows between two block shapes (RowList::next2)
     int next_num2[kNumBins];
+    // numeric long-row bin: rows of more than kLightRowMax entries (counted
+    // by the scan) and the numeric scatter's counter for them
+    int num_heavy;
+    int heavy_fill;
 };
 
 struct CsrIn {
@@ -128,12 +132,19 @@ inline int row_grid(const RowList &rl, int max_blocks)
     return rl.dsize ? (rl.n < max_blocks ? rl.n : max_blocks) : rl.n;
 }
 
+// Rows of the numeric long-row bin with at most this many entries run in
+// the kernel's 512-thread shape, the others in its 1024-thread shape.
+constexpr long long kLightRowMax = 131072;
+
 // --- setup / binning / scan (k_setup_scan.hip) ---
 void launch_zero_stats(DevStats *st, hipStream_t s);
 // row_flop[M] (int32, saturated), per-bin counts; rows with f == 0 get row_nnz = 0.
 void launch_setup(const CsrIn &A, const CsrIn &B, int *row_flop, int *row_nnz, DevStats *st, const BinBounds &bb,
                   hipStream_t s);
-// Scatters row ids into bins (which = 0: symbolic by row_flop, 1: numeric by row_nnz).
+// Scatters row ids into bins (which = 0: symbolic by row_flop, 1: numeric by
+// row_nnz).  Numeric: the long-row bin's list holds its rows of at most
+// kLightRowMax entries first and the DevStats::num_heavy others after them
+// (the scan that counted them runs first).
 void launch_bin_scatter(const int *key, int M, int which, DevStats *st, int *bins, const BinBounds &bb,
                         hipStream_t s);
 // Exclusive scan of row_nnz into crpt64 (and crpt32 when nnz(C) fits int32
@@ -215,6 +226,11 @@ struct LongScratch {
     // sub-window's end in each B row.
     const int *ucut = nullptr;
     int nunit = 0;
+    // With a host-sized row list: its last nheavy rows hold more than
+    // kLightRowMax entries and the others at most that (the numeric
+    // scatter's order), so each block shape launches over its own rows;
+    // -1: unknown (each launch passes over the other's rows).
+    int nheavy = -1;
 };
 
 // wcut[k * (nwin + 1) + w] = first entry of B row k with column >= w *

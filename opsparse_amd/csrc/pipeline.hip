@@ -458,6 +458,7 @@ void symbolic_phase(ops_workspace *ws, const CsrIn &A, const CsrIn &B, PlanCore 
         CK(hipGetLastError());
         pc.bins = bins_dst;
     }
+    pc.ls.nheavy = pc.bins ? st.num_heavy : -1;
     if (sep) {
         CK(hipStreamSynchronize(s));
         t->numeric_binning = now_s() - t0;
