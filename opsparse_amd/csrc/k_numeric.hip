@@ -1196,8 +1196,7 @@ void launch_numeric_bin(int bin, const CsrIn &A, const CsrIn &B, const RowList &
                 rh.rows = rl.rows + rlo.n;
                 rh.n = ls.nheavy;
             }
-            // (device-sized lists: two 512-thread blocks per CU, one 1024-thread block)
-            const int grid = row_grid(rlo, 2 * kNumCu), hgrid = row_grid(rh, kNumCu);
+            const int grid = row_grid(rlo, kNumCu), hgrid = row_grid(rh, kNumCu);
             if (ls.off32) {
                 if (lm >= 0 && rlo.n > 0)
                     k_num_rank<V, true, kLightBS><<<grid, kLightBS, RankWin<V, kLightBS>::lds, s>>>(
